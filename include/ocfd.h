@@ -1,0 +1,214 @@
+/*
+ * ocfd.h - C ABI of the MI355X (gfx950) right-hand-side path of OpenCFD-SCU.
+ *
+ * Two layers are exported by libocfd_hip.so:
+ *
+ *  1. Reference-named entry points with the same signatures and meaning as the
+ *     reference's extern "C" functions.  They act on one process-wide context,
+ *     so the reference's own control flow (opencfd.c) can call them unchanged.
+ *
+ *  2. An explicit context API (ocfd_*) underneath them: the context owns every
+ *     device buffer, the caller owns every host array, each call returns an int
+ *     status (0 = ok), one HIP stream per context, no hidden host threads, not
+ *     thread-safe per context.
+ *
+ * Host arrays at the boundary are FP64 in the reference's file order: x fastest,
+ * then y, then z, interior points only (no ghost layers).  Internally the
+ * library keeps the reference's LAP=4 ghost layout on the GPU.
+ *
+ * Citations are to the reference tree (OpenCFD-SCU); paths are relative to it.
+ */
+#ifndef OCFD_H
+#define OCFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCFD_LAP 4          /* head/config_parameters.h:15 */
+#define OCFD_NVARS 5        /* head/config_parameters.h:17 */
+#define OCFD_MAX_FILTERS 10 /* src/parameters.c:102 (NF_max) */
+
+/* Scheme identifiers, src/parameters.c:567-637. */
+enum {
+    OCFD_CD6 = 203, OCFD_CD8 = 204,
+    OCFD_UP7 = 301, OCFD_WENO5 = 302, OCFD_WENO7 = 303, OCFD_WENO7_SYMBO = 304,
+    OCFD_WENO7_SYMBO_LIM = 305, OCFD_NND2 = 306, OCFD_OMP6_HR = 307,
+    OCFD_OMP6_LD = 308, OCFD_OMP6_CD8 = 309, OCFD_HYBRIDAUTO = 310
+};
+
+/* Arithmetic of the bulk kernels. */
+enum { OCFD_PREC_FP64 = 0, OCFD_PREC_MIXED = 1 };
+
+/* Status codes. */
+enum {
+    OCFD_OK = 0, OCFD_ERR_ARG = 1, OCFD_ERR_CONFIG = 2, OCFD_ERR_IO = 3,
+    OCFD_ERR_HIP = 4, OCFD_ERR_UNSUPPORTED = 5, OCFD_ERR_STATE = 6
+};
+
+/*
+ * Contents of opencfd-scu.in (src/parameters.c:166-564).  Field names follow
+ * the input keys; comments give the key.
+ */
+typedef struct ocfd_config {
+    int nx, ny, nz;              /* GRID_3D                                  */
+    int npx0, npy0, npz0;        /* PARALLEL_3D (must be 1 1 1 here)         */
+    int stream_mode;             /* STREAM (0 supported)                     */
+    int charteric;               /* CHARTERIC (0 supported)                  */
+    int test;                    /* TEST                                     */
+    int iperiodic[3];            /* IPERIODIC                                */
+    int jac_bound[3];            /* JAC_BOUND                                */
+    int dif_bound[6];            /* DIF_BOUND  (x-, x+, y-, y+, z-, z+)      */
+    int non_ref[6];              /* NON_REFLETION (same order)               */
+    int scheme_invis;            /* SCHEME_INVIS -> 301..310                 */
+    int scheme_vis;              /* SCHEME_VIS   -> 203/204                  */
+    int hy_style;                /* HY_STYLE (2 supported)                   */
+    int hy_smooth_dp;            /* HY_SMOOTH_DP                             */
+    int hy_patch_zone;           /* HY_PATCH_ZONE                            */
+    double hy_dp_intv[2];        /* HY_DP_INTV                               */
+    double re, ama, gamma, pr;   /* RE AMA GAMMA PR                          */
+    double t_ref;                /* T_REF (<=0 -> 288.15)                    */
+    double epsl_sw;              /* EPSL_SW                                  */
+    double dt, end_time;         /* DT END_TIME                              */
+    int kstep_show, kstep_save;  /* KSTEP_SHOW KSTEP_SAVE                    */
+    int init_stat;               /* INIT_STAT                                */
+    int ibc;                     /* IBC                                      */
+    int n_bc_npara, n_bc_rpara;
+    int bc_npara[32];            /* BC_NPARA                                 */
+    double bc_rpara[32];         /* BC_RPARA                                 */
+    int nfilter;                 /* number of FILTER_NPARA& keys             */
+    int filter_npara[OCFD_MAX_FILTERS][11]; /* FILTER_NPARA&                */
+    double filter_rpara[OCFD_MAX_FILTERS][3]; /* FILTER_RPARA&              */
+    int precision;               /* OCFD_PREC_* (not an input key)           */
+} ocfd_config;
+
+typedef struct ocfd_ctx ocfd_ctx;
+
+/* ---- configuration / formats (host side) ------------------------------- */
+
+/* Parse an opencfd-scu.in file with the reference's key rules
+ * (src/parameters.c:197-564, 667-792): spaces in keys are ignored, the last
+ * occurrence of a key wins, '#' lines never match, unknown keys are ignored,
+ * numbered FILTER_NPARA&/FILTER_RPARA& families are discovered by stripping
+ * digits.  Unset keys are 0.  Returns OCFD_ERR_CONFIG on a bad scheme name
+ * (the reference exits, src/parameters.c:661-664). */
+int ocfd_read_config(const char *path, ocfd_config *cfg);
+
+/* Flow file (opencfd.dat / OCFD%08d.dat): header record {int32 Istep, f64 tt}
+ * then d,u,v,w,T each as nz Fortran records of nx*ny f64
+ * (src/OCFD_IO.c:24-228).  Arrays are nx*ny*nz, x fastest. */
+int ocfd_read_flow_file(const char *path, int nx, int ny, int nz, double *d,
+                        double *u, double *v, double *w, double *T,
+                        int *istep, double *tt);
+int ocfd_write_flow_file(const char *path, int nx, int ny, int nz,
+                         const double *d, const double *u, const double *v,
+                         const double *w, const double *T, int istep,
+                         double tt);
+/* nfield 3-D arrays written/read back to back as Fortran records
+ * (OCFD3d-Mesh.dat: 3 fields; OCFD3d-Jacobi.dat: 13 fields,
+ * src/OCFD_Comput_Jacobian3d.cu:122-215). */
+int ocfd_read_records(const char *path, int nx, int ny, int nz, int nfield,
+                      double *out);
+int ocfd_write_records(const char *path, int nx, int ny, int nz, int nfield,
+                       const double *in);
+
+/* ---- explicit context API ---------------------------------------------- */
+
+/* Allocate a context on HIP device `device`.  Replaces opencfd_mem_init_all
+ * (src/OCFD_init.cu:134-235, 414-488) and opencfd_para_init
+ * (src/OCFD_init.cu:381-410). */
+int ocfd_create(const ocfd_config *cfg, int device, ocfd_ctx **out);
+int ocfd_destroy(ocfd_ctx *ctx);
+const char *ocfd_last_error(void);
+
+/* Grid: mesh coordinates x,y,z (interior, nx*ny*nz each).  Metrics are
+ * computed on the GPU as Init_Jacobian3d/comput_Jac3d do
+ * (src/OCFD_Comput_Jacobian3d.cu:30-424). */
+int ocfd_load_mesh(ocfd_ctx *ctx, const double *x, const double *y,
+                   const double *z);
+/* Grid: the 13 arrays of OCFD3d-Jacobi.dat in file order Axx Ayy Azz Akx Aky
+ * Akz Aix Aiy Aiz Asx Asy Asz Ajac, back to back (13*nx*ny*nz). */
+int ocfd_load_jacobi(ocfd_ctx *ctx, const double *arrays13);
+
+/* Flow: primitive variables (interior).  Replaces read_file+memcpy_inner
+ * (src/OCFD_IO.c:24-91, src/OCFD_init.cu:83-97). */
+int ocfd_load_flow(ocfd_ctx *ctx, const double *d, const double *u,
+                   const double *v, const double *w, const double *T,
+                   int istep, double tt);
+/* Inlet profile for IBC 108: ny rows of (d,u,v,T) as 4*ny doubles laid out
+ * [var][j] (flow1d-inlet.dat, src/OCFD_boundary_init.c:376-395).  Optional:
+ * without it the uniform (1,1,0,1) profile is used when BC_NPARA[2] != 1. */
+int ocfd_set_inlet_profile(ocfd_ctx *ctx, const double *prof4xny);
+
+/* Finish initialisation: prim->cons, boundary-condition set-up, the initial
+ * ghost fill, BC and viscosity of NS_solver_real (src/OCFD_init.cu:99-131,
+ * src/OCFD_NS_Solver.cu:46-54). */
+int ocfd_init(ocfd_ctx *ctx);
+
+/* One right-hand side evaluation into the context's du (du_comput,
+ * src/OCFD_Stream.cu:28-54). */
+int ocfd_rhs(ocfd_ctx *ctx, int krk);
+/* nsteps full time steps (body of NS_solver_real, src/OCFD_NS_Solver.cu:63-167,
+ * without show/save/analysis). */
+int ocfd_step(ocfd_ctx *ctx, int nsteps);
+int ocfd_sync(ocfd_ctx *ctx);
+
+/* Copies back to the host (interior, x fastest). */
+int ocfd_get_flow(ocfd_ctx *ctx, double *d, double *u, double *v, double *w,
+                  double *T, int *istep, double *tt);
+int ocfd_get_state(ocfd_ctx *ctx, double *f5);   /* conserved, [5][nz][ny][nx] */
+int ocfd_get_du(ocfd_ctx *ctx, double *du5);     /* last RHS, same layout */
+int ocfd_get_pressure(ocfd_ctx *ctx, double *p); /* interior p */
+int ocfd_get_amu(ocfd_ctx *ctx, double *amu);
+/* 13 metric arrays in OCFD3d-Jacobi.dat order, interior. */
+int ocfd_get_metrics(ocfd_ctx *ctx, double *arrays13);
+/* hybrid_auto flags: x (nx+1)*ny*nz, y nx*(ny+1)*nz, z nx*ny*(nz+1). */
+int ocfd_get_hybrid_flags(ocfd_ctx *ctx, int8_t *fx, int8_t *fy, int8_t *fz);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the
+ * context's stream and accumulated per kernel name. */
+int ocfd_profile_enable(ocfd_ctx *ctx, int on);
+int ocfd_profile_count(ocfd_ctx *ctx);
+/* i-th entry: name (NUL-terminated, <=63 chars), launches, total ms. */
+int ocfd_profile_entry(ocfd_ctx *ctx, int i, char *name, int *launches,
+                       double *total_ms);
+int ocfd_profile_reset(ocfd_ctx *ctx);
+/* Wall time of the last ocfd_step call measured with HIP events, ms. */
+double ocfd_last_step_ms(ocfd_ctx *ctx);
+
+/* ---- reference-named entry points (process-wide context) -------------- */
+
+/* read_parameters (src/parameters.c:197): reads ./opencfd-scu.in. */
+void read_parameters(void);
+/* opencfd_mem_init_all (src/OCFD_init.cu:134). */
+void opencfd_mem_init_all(void);
+/* init (src/OCFD_init.cu:26): metrics from ./OCFD3d-Jacobi.dat or
+ * ./OCFD3d-Mesh.dat, flow from Opencfd.msg/opencfd.dat, IBC set-up. */
+void init(void);
+/* NS_solver_real (src/OCFD_NS_Solver.cu:39): the time loop to END_TIME with
+ * KSTEP_SAVE output files. */
+void NS_solver_real(void);
+/* du_comput (src/OCFD_Stream.cu:28). */
+void du_comput(int KRK);
+/* OCFD_time_advance (src/OCFD_time.cu:96). */
+void OCFD_time_advance(int KRK);
+/* get_duvwT (src/commen_kernel.cu:111). */
+void get_duvwT(void);
+/* get_Amu (src/commen_kernel.cu:133). */
+void get_Amu(void);
+/* OCFD_bc (src/OCFD_boundary.cu:18). */
+void OCFD_bc(void);
+/* filtering (src/OCFD_filtering.cu:75); the pointers are the reference's host
+ * mirrors and are not dereferenced. */
+void filtering(double *pf, double *pf0, double *pp);
+/* opencfd_mem_finalize_all (src/OCFD_init.cu:238). */
+void opencfd_mem_finalize_all(void);
+/* Access to the process-wide context used by the functions above. */
+ocfd_ctx *ocfd_global_ctx(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCFD_H */

@@ -1,0 +1,324 @@
+"""ctypes bindings of libocfd_hip.so (declared in include/ocfd.h).
+
+This module is the Python host-side mirror of the C ABI: the ``OcfdConfig``
+structure mirrors ``ocfd_config`` and :class:`Solver` wraps one ``ocfd_ctx``.
+All compute runs in the HIP library; loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent          # opencfd-scu_amd/
+LIB_PATH = PKG_DIR / "libocfd_hip.so"
+
+LAP = 4
+NVARS = 5
+MAX_FILTERS = 10
+
+CD6, CD8 = 203, 204
+UP7, WENO5, WENO7, WENO7_SYMBO, WENO7_SYMBO_LIM, NND2 = 301, 302, 303, 304, 305, 306
+OMP6_HR, OMP6_LD, OMP6_CD8, HYBRIDAUTO = 307, 308, 309, 310
+PREC_FP64, PREC_MIXED = 0, 1
+
+SCHEME_IDS = {
+    "UP7": UP7, "WENO5": WENO5, "WENO7": WENO7, "WENO7_SYMBO": WENO7_SYMBO,
+    "WENO7_SYMBO_LIM": WENO7_SYMBO_LIM, "NND2": NND2, "OMP6_HR": OMP6_HR,
+    "OMP6_LD": OMP6_LD, "OMP6_CD8": OMP6_CD8, "SCHEME_HYBRIDAUTO": HYBRIDAUTO,
+    "CD6": CD6, "CD8": CD8,
+}
+
+
+class OcfdConfig(C.Structure):
+    """Mirror of ``ocfd_config`` (include/ocfd.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("npx0", C.c_int), ("npy0", C.c_int), ("npz0", C.c_int),
+        ("stream_mode", C.c_int), ("charteric", C.c_int), ("test", C.c_int),
+        ("iperiodic", C.c_int * 3), ("jac_bound", C.c_int * 3),
+        ("dif_bound", C.c_int * 6), ("non_ref", C.c_int * 6),
+        ("scheme_invis", C.c_int), ("scheme_vis", C.c_int),
+        ("hy_style", C.c_int), ("hy_smooth_dp", C.c_int), ("hy_patch_zone", C.c_int),
+        ("hy_dp_intv", C.c_double * 2),
+        ("re", C.c_double), ("ama", C.c_double), ("gamma", C.c_double), ("pr", C.c_double),
+        ("t_ref", C.c_double), ("epsl_sw", C.c_double),
+        ("dt", C.c_double), ("end_time", C.c_double),
+        ("kstep_show", C.c_int), ("kstep_save", C.c_int), ("init_stat", C.c_int),
+        ("ibc", C.c_int),
+        ("n_bc_npara", C.c_int), ("n_bc_rpara", C.c_int),
+        ("bc_npara", C.c_int * 32), ("bc_rpara", C.c_double * 32),
+        ("nfilter", C.c_int),
+        ("filter_npara", (C.c_int * 11) * MAX_FILTERS),
+        ("filter_rpara", (C.c_double * 3) * MAX_FILTERS),
+        ("precision", C.c_int),
+    ]
+
+
+def make_config(**kw) -> OcfdConfig:
+    """Build an ``OcfdConfig`` from keyword arguments (field names above).
+
+    Array fields accept sequences; ``filters`` is a list of
+    ``(npara11, rpara3)`` pairs; ``bc_npara``/``bc_rpara`` set their counts.
+    """
+    c = OcfdConfig()
+    c.npx0 = c.npy0 = c.npz0 = 1
+    c.gamma = 1.4
+    c.pr = 0.72
+    c.t_ref = 288.15
+    c.scheme_vis = CD6
+    c.scheme_invis = WENO7
+    c.hy_style = 2
+    c.kstep_show = 1
+    c.kstep_save = 1 << 30
+    c.init_stat = 1
+    filters = kw.pop("filters", [])
+    for k, v in kw.items():
+        if k in ("iperiodic", "jac_bound", "dif_bound", "non_ref", "hy_dp_intv"):
+            arr = getattr(c, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k == "bc_npara":
+            for i, x in enumerate(v):
+                c.bc_npara[i] = int(x)
+            c.n_bc_npara = len(v)
+        elif k == "bc_rpara":
+            for i, x in enumerate(v):
+                c.bc_rpara[i] = float(x)
+            c.n_bc_rpara = len(v)
+        elif k in ("scheme_invis", "scheme_vis") and isinstance(v, str):
+            setattr(c, k, SCHEME_IDS[v])
+        else:
+            setattr(c, k, v)
+    c.nfilter = len(filters)
+    for m, (npara, rpara) in enumerate(filters):
+        for i in range(11):
+            c.filter_npara[m][i] = int(npara[i])
+        for i in range(3):
+            c.filter_rpara[m][i] = float(rpara[i])
+    return c
+
+
+def config_to_dict(c: OcfdConfig) -> dict:
+    out = {}
+    for name, _ in OcfdConfig._fields_:
+        v = getattr(c, name)
+        if hasattr(v, "__len__"):
+            v = [list(x) if hasattr(x, "__len__") else x for x in v]
+        out[name] = v
+    return out
+
+
+_lib = None
+
+
+def load_library(path: os.PathLike | None = None) -> C.CDLL:
+    """Load libocfd_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build it with `python __graft_entry__.py build` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    lib = C.CDLL(str(p))
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    sig = {
+        "ocfd_read_config": (C.c_int, [C.c_char_p, C.POINTER(OcfdConfig)]),
+        "ocfd_read_flow_file": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp,
+                                          C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ocfd_write_flow_file": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp,
+                                           C.c_int, C.c_double]),
+        "ocfd_read_records": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+        "ocfd_write_records": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp]),
+        "ocfd_create": (C.c_int, [C.POINTER(OcfdConfig), C.c_int, C.POINTER(vp)]),
+        "ocfd_destroy": (C.c_int, [vp]),
+        "ocfd_last_error": (C.c_char_p, []),
+        "ocfd_load_mesh": (C.c_int, [vp, dp, dp, dp]),
+        "ocfd_load_jacobi": (C.c_int, [vp, dp]),
+        "ocfd_load_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.c_int, C.c_double]),
+        "ocfd_set_inlet_profile": (C.c_int, [vp, dp]),
+        "ocfd_init": (C.c_int, [vp]),
+        "ocfd_rhs": (C.c_int, [vp, C.c_int]),
+        "ocfd_step": (C.c_int, [vp, C.c_int]),
+        "ocfd_sync": (C.c_int, [vp]),
+        "ocfd_get_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ocfd_get_state": (C.c_int, [vp, dp]),
+        "ocfd_get_du": (C.c_int, [vp, dp]),
+        "ocfd_get_pressure": (C.c_int, [vp, dp]),
+        "ocfd_get_amu": (C.c_int, [vp, dp]),
+        "ocfd_get_metrics": (C.c_int, [vp, dp]),
+        "ocfd_get_hybrid_flags": (C.c_int, [vp, i8p, i8p, i8p]),
+        "ocfd_profile_enable": (C.c_int, [vp, C.c_int]),
+        "ocfd_profile_count": (C.c_int, [vp]),
+        "ocfd_profile_entry": (C.c_int, [vp, C.c_int, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ocfd_profile_reset": (C.c_int, [vp]),
+        "ocfd_last_step_ms": (C.c_double, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = load_library().ocfd_last_error()
+        raise RuntimeError(f"{what} failed (status {rc}): {msg.decode() if msg else ''}")
+
+
+def read_config(path: os.PathLike) -> OcfdConfig:
+    c = OcfdConfig()
+    _check(load_library().ocfd_read_config(str(path).encode(), C.byref(c)), "ocfd_read_config")
+    return c
+
+
+def read_flow_file(path, nx, ny, nz):
+    arrs = [np.empty(nx * ny * nz) for _ in range(5)]
+    istep, tt = C.c_int(), C.c_double()
+    _check(load_library().ocfd_read_flow_file(str(path).encode(), nx, ny, nz, *arrs,
+                                              C.byref(istep), C.byref(tt)), "read_flow_file")
+    return [a.reshape(nz, ny, nx) for a in arrs], istep.value, tt.value
+
+
+def write_flow_file(path, prims, istep, tt):
+    nz, ny, nx = prims[0].shape
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in prims]
+    _check(load_library().ocfd_write_flow_file(str(path).encode(), nx, ny, nz, *arrs, int(istep),
+                                               float(tt)), "write_flow_file")
+
+
+def write_records(path, arrays):
+    nz, ny, nx = arrays[0].shape
+    buf = np.ascontiguousarray(np.stack(arrays), dtype=np.float64).ravel()
+    _check(load_library().ocfd_write_records(str(path).encode(), nx, ny, nz, len(arrays), buf),
+           "write_records")
+
+
+def read_records(path, nx, ny, nz, nfield):
+    buf = np.empty(nfield * nx * ny * nz)
+    _check(load_library().ocfd_read_records(str(path).encode(), nx, ny, nz, nfield, buf), "read_records")
+    return buf.reshape(nfield, nz, ny, nx)
+
+
+class Solver:
+    """One ``ocfd_ctx`` on a HIP device.  Arrays are (nz, ny, nx) float64."""
+
+    def __init__(self, cfg: OcfdConfig, device: int = 0):
+        self.lib = load_library()
+        self.cfg = cfg
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        h = C.c_void_p()
+        _check(self.lib.ocfd_create(C.byref(cfg), device, C.byref(h)), "ocfd_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ocfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def _flat(self, a):
+        return np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+    def load_mesh(self, x, y, z):
+        _check(self.lib.ocfd_load_mesh(self.h, self._flat(x), self._flat(y), self._flat(z)), "load_mesh")
+
+    def load_jacobi(self, a13):
+        _check(self.lib.ocfd_load_jacobi(self.h, self._flat(a13)), "load_jacobi")
+
+    def load_flow(self, d, u, v, w, T, istep=0, tt=0.0):
+        _check(self.lib.ocfd_load_flow(self.h, *(self._flat(a) for a in (d, u, v, w, T)), int(istep),
+                                       float(tt)), "load_flow")
+
+    def set_inlet_profile(self, prof):
+        _check(self.lib.ocfd_set_inlet_profile(self.h, self._flat(prof)), "set_inlet_profile")
+
+    def init(self):
+        _check(self.lib.ocfd_init(self.h), "ocfd_init")
+
+    def rhs(self, krk=1):
+        _check(self.lib.ocfd_rhs(self.h, krk), "ocfd_rhs")
+
+    def step(self, n=1):
+        _check(self.lib.ocfd_step(self.h, n), "ocfd_step")
+
+    def sync(self):
+        _check(self.lib.ocfd_sync(self.h), "ocfd_sync")
+
+    def get_flow(self):
+        n = self.nx * self.ny * self.nz
+        arrs = [np.empty(n) for _ in range(5)]
+        istep, tt = C.c_int(), C.c_double()
+        _check(self.lib.ocfd_get_flow(self.h, *arrs, C.byref(istep), C.byref(tt)), "get_flow")
+        return [a.reshape(self.shape) for a in arrs], istep.value, tt.value
+
+    def get_state(self):
+        out = np.empty(5 * self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_state(self.h, out), "get_state")
+        return out.reshape((5,) + self.shape)
+
+    def get_du(self):
+        out = np.empty(5 * self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_du(self.h, out), "get_du")
+        return out.reshape((5,) + self.shape)
+
+    def get_pressure(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_pressure(self.h, out), "get_pressure")
+        return out.reshape(self.shape)
+
+    def get_amu(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_amu(self.h, out), "get_amu")
+        return out.reshape(self.shape)
+
+    def get_metrics(self):
+        out = np.empty(13 * self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_metrics(self.h, out), "get_metrics")
+        return out.reshape((13,) + self.shape)
+
+    def get_flags(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        fx = np.empty((nx + 1) * ny * nz, np.int8)
+        fy = np.empty(nx * (ny + 1) * nz, np.int8)
+        fz = np.empty(nx * ny * (nz + 1), np.int8)
+        _check(self.lib.ocfd_get_hybrid_flags(self.h, fx, fy, fz), "get_hybrid_flags")
+        return fx.reshape(nz, ny, nx + 1), fy.reshape(nz, ny + 1, nx), fz.reshape(nz + 1, ny, nx)
+
+    # -- kernel timing -----------------------------------------------------
+    def profile(self, on=True):
+        _check(self.lib.ocfd_profile_enable(self.h, 1 if on else 0), "profile_enable")
+
+    def profile_reset(self):
+        _check(self.lib.ocfd_profile_reset(self.h), "profile_reset")
+
+    def profile_entries(self):
+        out = {}
+        buf = C.create_string_buffer(64)
+        for i in range(self.lib.ocfd_profile_count(self.h)):
+            n, ms = C.c_int(), C.c_double()
+            _check(self.lib.ocfd_profile_entry(self.h, i, buf, C.byref(n), C.byref(ms)), "profile_entry")
+            out[buf.value.decode()] = (n.value, ms.value)
+        return out
+
+    def last_step_ms(self):
+        return self.lib.ocfd_last_step_ms(self.h)

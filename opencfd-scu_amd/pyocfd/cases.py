@@ -1,0 +1,289 @@
+"""Case generators for the benchmark configurations (synthetic, seeded).
+
+The reference ships its own mesh/initial-field generators as separate C
+programs (benchmark_comp/swept-compression-ramp.c, init3d-inlet1d.c); these are
+this project's own generators for the workloads it measures:
+
+* ``tgv``   - 3-D Taylor-Green vortex in a periodic box with identity metrics.
+* ``ramp``  - 24-degree compression ramp (optionally swept in z) on a
+              tanh-stretched wall-normal grid, IBC 108 inflow/wall, hybrid
+              WENO7/UP7 + CD8, the shock-capturing filter.
+
+Each generator returns a :class:`Case` with the config, the grid (mesh
+coordinates or the 13 Jacobi arrays), the primitive fields and, for the ramp,
+the inlet profile.  ``Case.write(dir)`` writes the reference's input files
+(opencfd-scu.in, OCFD3d-Mesh.dat / OCFD3d-Jacobi.dat, opencfd.dat,
+flow1d-inlet.dat).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import capi
+
+SEED = 20261202
+
+
+@dataclass
+class Case:
+    name: str
+    cfg: capi.OcfdConfig
+    prims: list                       # d,u,v,w,T each (nz,ny,nx)
+    mesh: tuple | None = None         # x,y,z each (nz,ny,nx)
+    jacobi: np.ndarray | None = None  # (13,nz,ny,nx)
+    inlet: np.ndarray | None = None   # (4,ny): d,u,v,T
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def npoints(self):
+        return self.cfg.nx * self.cfg.ny * self.cfg.nz
+
+    def load(self, solver, init=True):
+        """Push grid, flow and inlet profile into a Solver/Oracle and init it."""
+        if self.mesh is not None:
+            solver.load_mesh(*self.mesh)
+        else:
+            solver.load_jacobi(self.jacobi)
+        solver.load_flow(*self.prims, istep=0, tt=0.0)
+        if self.inlet is not None:
+            solver.set_inlet_profile(self.inlet)
+        if init:
+            solver.init()
+        return solver
+
+    def input_text(self) -> str:
+        return config_text(self.cfg)
+
+    def write(self, d: Path):
+        """Write the reference's input files for this case into directory d."""
+        d = Path(d)
+        d.mkdir(parents=True, exist_ok=True)
+        (d / "opencfd-scu.in").write_text(self.input_text())
+        if self.mesh is not None:
+            capi.write_records(d / "OCFD3d-Mesh.dat", list(self.mesh))
+        else:
+            capi.write_records(d / "OCFD3d-Jacobi.dat", list(self.jacobi))
+        capi.write_flow_file(d / "opencfd.dat", self.prims, 0, 0.0)
+        if self.inlet is not None:
+            ny = self.cfg.ny
+            y = self.extra.get("inlet_y", np.arange(ny, dtype=float))
+            with open(d / "flow1d-inlet.dat", "w") as fh:
+                fh.write(" variables=y,d,u,v,T\n")
+                for j in range(ny):
+                    fh.write("%24.16e %24.16e %24.16e %24.16e %24.16e\n" %
+                             (y[j], self.inlet[0, j], self.inlet[1, j], self.inlet[2, j], self.inlet[3, j]))
+
+
+def config_text(c: capi.OcfdConfig) -> str:
+    """opencfd-scu.in text for a config (keys of src/parameters.c:166-194)."""
+    inv = {v: k for k, v in capi.SCHEME_IDS.items()}
+    L = [
+        "# OpenCFD-SCU input file written by pyocfd.cases",
+        f"GRID_3D = {c.nx} {c.ny} {c.nz}",
+        f"PARALLEL_3D = {c.npx0 or 1} {c.npy0 or 1} {c.npz0 or 1}",
+        f"STREAM = {c.stream_mode}",
+        f"CHARTERIC = {c.charteric}",
+        f"TEST = {c.test}",
+        "IPERIODIC = %d %d %d" % tuple(c.iperiodic),
+        "JAC_BOUND = %d %d %d" % tuple(c.jac_bound),
+        "DIF_BOUND = %d %d %d %d %d %d" % tuple(c.dif_bound),
+        "NON_REFLETION = %d %d %d %d %d %d" % tuple(c.non_ref),
+        f"SCHEME_INVIS = {inv[c.scheme_invis]}",
+        f"SCHEME_VIS = {inv[c.scheme_vis]}",
+        "HY_DP_INTV = %.17g %.17g" % tuple(c.hy_dp_intv),
+        f"HY_SMOOTH_DP = {c.hy_smooth_dp}",
+        f"HY_STYLE = {c.hy_style}",
+        f"HY_PATCH_ZONE = {c.hy_patch_zone}",
+        f"RE = {c.re!r}",
+        f"AMA = {c.ama!r}",
+        f"GAMMA = {c.gamma!r}",
+        f"PR = {c.pr!r}",
+        f"T_REF = {c.t_ref!r}",
+        f"EPSL_SW = {c.epsl_sw!r}",
+        f"DT = {c.dt!r}",
+        f"END_TIME = {c.end_time!r}",
+        f"KSTEP_SHOW = {c.kstep_show}",
+        f"KSTEP_SAVE = {c.kstep_save}",
+        f"INIT_STAT = {c.init_stat}",
+        f"IBC = {c.ibc}",
+    ]
+    if c.n_bc_npara:
+        L.append("BC_NPARA = " + " ".join(str(c.bc_npara[i]) for i in range(c.n_bc_npara)))
+    if c.n_bc_rpara:
+        L.append("BC_RPARA = " + " ".join(repr(c.bc_rpara[i]) for i in range(c.n_bc_rpara)))
+    for m in range(c.nfilter):
+        L.append(f"FILTER_NPARA{m} = " + " ".join(str(c.filter_npara[m][i]) for i in range(11)))
+        L.append(f"FILTER_RPARA{m} = " + " ".join(repr(c.filter_rpara[m][i]) for i in range(3)))
+    return "\n".join(L) + "\n"
+
+
+def end_time_for(nsteps: int, dt: float) -> float:
+    """END_TIME giving exactly nsteps with the reference's while(tt<end_time)."""
+    return (nsteps - 0.5) * dt
+
+
+# --------------------------------------------------------------------------
+# Taylor-Green vortex
+# --------------------------------------------------------------------------
+def tgv(n=64, ma=0.1, re=1600.0, noise=1e-4, seed=SEED, cfl=0.5, scheme="WENO7",
+        scheme_vis="CD6", nsteps=200, precision=capi.PREC_FP64) -> Case:
+    """Periodic TGV box, identity Jacobian (Akx=Aiy=Asz=1, Ajac=1).
+
+    With h = 1/(N-1) (src/OCFD_init.cu:29-31) the periodic length of the box
+    in computational units is L = N*h; the vortex has wavenumber 2*pi/L.
+    """
+    nx = ny = nz = n
+    gamma = 1.4
+    h = 1.0 / (n - 1.0)
+    L = n * h
+    k = 2.0 * math.pi / L
+    xs = np.arange(n) * h
+    z3, y3, x3 = np.meshgrid(xs, xs, xs, indexing="ij")
+    rng = np.random.default_rng(seed)
+    u = np.sin(k * x3) * np.cos(k * y3) * np.cos(k * z3)
+    v = -np.cos(k * x3) * np.sin(k * y3) * np.cos(k * z3)
+    w = np.zeros_like(u)
+    if noise:
+        u = u + noise * rng.standard_normal(u.shape)
+        v = v + noise * rng.standard_normal(v.shape)
+        w = w + noise * rng.standard_normal(w.shape)
+    d = np.ones_like(u)
+    p = 1.0 / (gamma * ma * ma) + (np.cos(2 * k * x3) + np.cos(2 * k * y3)) * (np.cos(2 * k * z3) + 2.0) / 16.0
+    T = gamma * ma * ma * p / d
+    # CFL 0.5 on the inviscid spectral radius (computational units)
+    c = np.sqrt(T) / ma
+    lam = (np.abs(u) + c) / h + (np.abs(v) + c) / h + (np.abs(w) + c) / h
+    dt = float(cfl / lam.max())
+    jac = np.zeros((13, nz, ny, nx))
+    jac[0], jac[1], jac[2] = x3, y3, z3
+    jac[3] = 1.0   # Akx
+    jac[7] = 1.0   # Aiy
+    jac[11] = 1.0  # Asz
+    jac[12] = 1.0  # Ajac
+    cfg = capi.make_config(
+        nx=nx, ny=ny, nz=nz, iperiodic=(1, 1, 1), jac_bound=(1, 1, 1),
+        dif_bound=(0,) * 6, non_ref=(0,) * 6, scheme_invis=scheme, scheme_vis=scheme_vis,
+        re=re, ama=ma, gamma=gamma, pr=0.72, t_ref=288.15, epsl_sw=0.0, dt=dt,
+        end_time=end_time_for(nsteps, dt), ibc=0, precision=precision,
+        hy_dp_intv=(0.01, 0.2), hy_style=2)
+    return Case("tgv%d" % n, cfg, [d, u, v, w, T], jacobi=jac, extra={"nsteps": nsteps})
+
+
+# --------------------------------------------------------------------------
+# Compression ramp (2-D extruded or swept)
+# --------------------------------------------------------------------------
+def _tanh_stretch(ny, first):
+    """s_j in [0,1], s_0=0, s_1=first, tanh clustering at the wall."""
+    j = np.arange(ny) / (ny - 1.0)
+
+    def first_cell(b):
+        return 1.0 - np.tanh(b * (1.0 - j[1])) / np.tanh(b)
+
+    lo, hi = 1e-6, 50.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first_cell(mid) > first:
+            lo = mid
+        else:
+            hi = mid
+    b = 0.5 * (lo + hi)
+    return 1.0 - np.tanh(b * (1.0 - j)) / np.tanh(b)
+
+
+def ramp_mesh(nx, ny, nz, angle_deg=24.0, sweep_deg=0.0, height=0.5, span=0.1, first=1e-4,
+              corner=0.5, arc=0.02):
+    """Wall follows y_w(x): flat to x=corner, then angle_deg, joined by a
+    quadratic blend of half-width `arc`; wall-normal lines vertical; top flat at
+    y=height; z uniform over `span` (periodic); sweep shears x by z*tan(sweep).
+    """
+    t = math.tan(math.radians(angle_deg))
+    xs = np.linspace(0.0, 1.0, nx)
+    yw = np.where(xs < corner - arc, 0.0,
+                  np.where(xs > corner + arc, (xs - corner) * t,
+                           t * (xs - corner + arc) ** 2 / (4 * arc)))
+    s = _tanh_stretch(ny, first)
+    zs = np.arange(nz) * (span / nz)
+    X2 = np.broadcast_to(xs[None, :], (ny, nx))
+    Y2 = yw[None, :] + (height - yw[None, :]) * s[:, None]
+    x = np.empty((nz, ny, nx)); y = np.empty((nz, ny, nx)); z = np.empty((nz, ny, nx))
+    shear = math.tan(math.radians(sweep_deg))
+    for k in range(nz):
+        x[k] = X2 + zs[k] * shear
+        y[k] = Y2
+        z[k] = zs[k]
+    return x, y, z, s * height
+
+
+def blasius_like(yn, delta, ma, gamma, tw, pr=0.72):
+    """Pohlhausen quartic velocity profile of thickness delta with a
+    Crocco-Busemann temperature, uniform pressure p = 1/(gamma Ma^2)."""
+    eta = np.clip(yn / delta, 0.0, 1.0)
+    U = 2 * eta - 2 * eta ** 3 + eta ** 4
+    r = math.sqrt(pr)
+    tr = 1.0 + r * 0.5 * (gamma - 1.0) * ma * ma
+    T = tw + (tr - tw) * U + (1.0 - tr) * U * U
+    d = 1.0 / T
+    return d, U, T
+
+
+def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=0.02, tw=2.5,
+         pert=0.01, seed=SEED, cfl=0.5, scheme="SCHEME_HYBRIDAUTO", filt=True, filter_every=10,
+         nsteps=1000, precision=capi.PREC_FP64, height=0.5, span=0.1, t_ref=108.1) -> Case:
+    gamma = 1.4
+    x, y, z, ycol = ramp_mesh(nx, ny, nz, angle, sweep, height=height, span=span)
+    # wall distance along each grid line (vertical lines)
+    yn = y - y[:, :1, :]
+    d1, U1, T1 = blasius_like(ycol, delta, ma, gamma, tw)
+    d, U, T = blasius_like(yn, delta, ma, gamma, tw)
+    # seeded perturbations: white noise smoothed over a few cells (periodic in
+    # z), scaled to a peak of `pert`, with an envelope that vanishes at the
+    # wall and peaks at y = delta
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    eta = yn / delta
+    env = eta * np.exp(1.0 - eta)
+    comps = []
+    for _ in range(3):
+        r = gaussian_filter(rng.standard_normal(U.shape), sigma=(2.0, 2.0, 3.0), mode=("wrap", "nearest", "nearest"))
+        comps.append(pert * r / max(np.abs(r).max(), 1e-300) * env)
+    u = U + comps[0]
+    v = comps[1]
+    w = comps[2]
+    inlet = np.stack([d1, U1, np.zeros(ny), T1])
+    # CFL from the inviscid spectral radius in computational space (estimate
+    # with CD2 metrics of the generated grid)
+    hx, hy, hz = 1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1)
+    xx = np.gradient(x, axis=2) / hx; yx = np.gradient(y, axis=2) / hx
+    xy = np.gradient(x, axis=1) / hy; yy = np.gradient(y, axis=1) / hy
+    zz = span / nz / hz
+    det = (xx * yy - xy * yx) * zz
+    c = np.sqrt(T) / ma
+    # contravariant velocities per unit computational length
+    kx, ky = yy * zz / det, -xy * zz / det
+    ex, ey = -yx * zz / det, xx * zz / det
+    sz = 1.0 / zz
+    lam = (np.abs(kx * u + ky * v) + c * np.hypot(kx, ky)) / hx + \
+          (np.abs(ex * u + ey * v) + c * np.hypot(ex, ey)) / hy + (np.abs(w * sz) + c * sz) / hz
+    # viscous spectral radius: 2*gamma/Pr * mu/rho * sum |grad xi|^2 / h^2
+    tsb = 110.4 / t_ref
+    mu = (1.0 + tsb) / re * T ** 1.5 / (tsb + T)
+    lam_v = 2.0 * gamma / 0.70 * mu / d * ((kx * kx + ky * ky) / hx ** 2 + (ex * ex + ey * ey) / hy ** 2
+                                          + (sz / hz) ** 2)
+    dt = float(cfl / (lam + lam_v).max())
+    filters = []
+    if filt:
+        filters = [((filter_every, 1, 1, 1, 0, nx, 0, ny, 0, nz, 2), (1.0, 1e-5, 1e9))]
+    cfg = capi.make_config(
+        nx=nx, ny=ny, nz=nz, iperiodic=(0, 0, 1), jac_bound=(1, 1, 1),
+        dif_bound=(0, 1, 0, 1, 0, 0), non_ref=(0, 1, 0, 1, 0, 0), scheme_invis=scheme, scheme_vis="CD8",
+        hy_style=2, hy_dp_intv=(0.01, 0.2), re=re, ama=ma, gamma=gamma, pr=0.70, t_ref=t_ref, epsl_sw=0.0,
+        dt=dt, end_time=end_time_for(nsteps, dt), ibc=108,
+        bc_npara=(10, 5, 1, 0),
+        bc_rpara=(tw, 0.02, 0.05, 0.1, 0.1, -1.0, -1.0, span),
+        filters=filters, precision=precision)
+    return Case("ramp%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z), inlet=inlet,
+                extra={"inlet_y": ycol, "nsteps": nsteps})

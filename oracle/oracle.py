@@ -1,0 +1,210 @@
+"""ctypes wrapper of the CPU oracle (oracle/ocfd_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+REPO = ORACLE_DIR.parent
+LIB = ORACLE_DIR / "liboracle.so"
+sys.path.insert(0, str(REPO / "opencfd-scu_amd"))
+from pyocfd.capi import OcfdConfig  # noqa: E402  (interface type only)
+
+
+def build(force: bool = False) -> Path:
+    """Compile liboracle.so with the committed Makefile."""
+    src = ORACLE_DIR / "ocfd_oracle.c"
+    if force or not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+        subprocess.run(["make", "-C", str(ORACLE_DIR), "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(str(LIB))
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        i8 = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+        vp = C.c_void_p
+        sig = {
+            "orc_create": (vp, [C.POINTER(OcfdConfig)]),
+            "orc_destroy": (None, [vp]),
+            "orc_load_mesh": (C.c_int, [vp, dp, dp, dp]),
+            "orc_load_jacobi": (C.c_int, [vp, dp]),
+            "orc_load_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.c_int, C.c_double]),
+            "orc_set_inlet_profile": (C.c_int, [vp, dp]),
+            "orc_init": (C.c_int, [vp]),
+            "orc_rhs": (C.c_int, [vp, C.c_int]),
+            "orc_step": (C.c_int, [vp, C.c_int]),
+            "orc_get_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+            "orc_get_state": (C.c_int, [vp, dp]),
+            "orc_get_du": (C.c_int, [vp, dp]),
+            "orc_get_pressure": (C.c_int, [vp, dp]),
+            "orc_get_amu": (C.c_int, [vp, dp]),
+            "orc_get_metrics": (C.c_int, [vp, dp]),
+            "orc_get_flags": (C.c_int, [vp, i8, i8, i8]),
+            "orc_face_flux": (C.c_double, [C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_double]),
+            "orc_split_point": (None, [C.c_double] * 10 + [dp]),
+            "orc_line_flux_derivative": (None, [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.c_int,
+                                                C.c_int, C.c_int, C.c_void_p, dp]),
+            "orc_line_central": (None, [C.c_int, C.c_int, dp, C.c_double, C.c_int, C.c_int, dp]),
+            "orc_jameson_dp": (C.c_double, [C.c_double] * 7),
+            "orc_rk_stage": (None, [C.c_int, C.c_int, C.c_double, dp, dp, dp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ip = ip  # noqa: F841
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """Mirror of pyocfd.capi.Solver backed by the CPU restatement."""
+
+    def __init__(self, cfg: OcfdConfig):
+        self.L = lib()
+        self.cfg = cfg
+        self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
+        self.h = self.L.orc_create(C.byref(cfg))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @staticmethod
+    def _flat(a):
+        return np.ascontiguousarray(a, dtype=np.float64).ravel()
+
+    def load_mesh(self, x, y, z):
+        self.L.orc_load_mesh(self.h, self._flat(x), self._flat(y), self._flat(z))
+
+    def load_jacobi(self, a13):
+        self.L.orc_load_jacobi(self.h, self._flat(a13))
+
+    def load_flow(self, d, u, v, w, T, istep=0, tt=0.0):
+        self.L.orc_load_flow(self.h, *(self._flat(a) for a in (d, u, v, w, T)), int(istep), float(tt))
+
+    def set_inlet_profile(self, prof):
+        self.L.orc_set_inlet_profile(self.h, self._flat(prof))
+
+    def init(self):
+        self.L.orc_init(self.h)
+
+    def rhs(self, krk=1):
+        self.L.orc_rhs(self.h, krk)
+
+    def step(self, n=1):
+        self.L.orc_step(self.h, n)
+
+    def get_flow(self):
+        n = self.nx * self.ny * self.nz
+        arrs = [np.empty(n) for _ in range(5)]
+        istep, tt = C.c_int(), C.c_double()
+        self.L.orc_get_flow(self.h, *arrs, C.byref(istep), C.byref(tt))
+        return [a.reshape(self.shape) for a in arrs], istep.value, tt.value
+
+    def get_state(self):
+        out = np.empty(5 * self.nx * self.ny * self.nz)
+        self.L.orc_get_state(self.h, out)
+        return out.reshape((5,) + self.shape)
+
+    def get_du(self):
+        out = np.empty(5 * self.nx * self.ny * self.nz)
+        self.L.orc_get_du(self.h, out)
+        return out.reshape((5,) + self.shape)
+
+    def get_pressure(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        self.L.orc_get_pressure(self.h, out)
+        return out.reshape(self.shape)
+
+    def get_amu(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        self.L.orc_get_amu(self.h, out)
+        return out.reshape(self.shape)
+
+    def get_metrics(self):
+        out = np.empty(13 * self.nx * self.ny * self.nz)
+        self.L.orc_get_metrics(self.h, out)
+        return out.reshape((13,) + self.shape)
+
+    def get_flags(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        fx = np.empty((nx + 1) * ny * nz, np.int8)
+        fy = np.empty(nx * (ny + 1) * nz, np.int8)
+        fz = np.empty(nx * ny * (nz + 1), np.int8)
+        self.L.orc_get_flags(self.h, fx, fy, fz)
+        return fx.reshape(nz, ny, nx + 1), fy.reshape(nz, ny + 1, nx), fz.reshape(nz + 1, ny, nx)
+
+
+# ---- unit-level helpers -------------------------------------------------
+def face_flux(scheme, side, s8, hyflag=0, tv_limiter=5.0, tv_max=0.2):
+    s = np.zeros(8)
+    s[: len(s8)] = s8
+    return lib().orc_face_flux(scheme, side, hyflag, s, tv_limiter, tv_max)
+
+
+def split_point(d, u, v, w, cc, A1, A2, A3, gamma=1.4, eps=0.0):
+    out = np.empty(10)
+    lib().orc_split_point(d, u, v, w, cc, A1, A2, A3, gamma, eps, out)
+    return out[:5], out[5:]
+
+
+def line_flux_derivative(scheme, side, f, ajac, h, bound_l=0, bound_r=0, nonref=0, flags=None):
+    """f has n+8 values (nodes -4..n+3); returns du (n) = -ajac*(h_r-h_l)/h."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = f.size - 8
+    aj = np.ascontiguousarray(ajac, dtype=np.float64)
+    du = np.zeros(n)
+    fl = None
+    if flags is not None:
+        fl_arr = np.ascontiguousarray(flags, dtype=np.int32)
+        fl = fl_arr.ctypes.data_as(C.c_void_p)
+    lib().orc_line_flux_derivative(scheme, side, n, f, aj, h, bound_l, bound_r, nonref, fl, du)
+    return du
+
+
+def line_central(scheme_vis, f, h, bound_l=0, bound_r=0):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = f.size - 8
+    out = np.zeros(n)
+    lib().orc_line_central(scheme_vis, n, f, h, bound_l, bound_r, out)
+    return out
+
+
+def rk_stage(krk, dt, f, fn, du):
+    f = np.ascontiguousarray(f, dtype=np.float64).copy()
+    lib().orc_rk_stage(f.size, krk, dt, f, np.ascontiguousarray(fn, dtype=np.float64),
+                       np.ascontiguousarray(du, dtype=np.float64))
+    return f
+
+
+def jameson_dp(pc, pxm, pxp, pym, pyp, pzm, pzp):
+    return lib().orc_jameson_dp(pc, pxm, pxp, pym, pyp, pzm, pzp)
