@@ -1,0 +1,156 @@
+#!/usr/bin/env python
+"""Throughput of the OpenCFD-SCU right-hand-side path on one MI355X.
+
+Workload (N=1): the 512x128x64 compression ramp - hybrid_auto WENO7/UP7
+inviscid fluxes on the curvilinear (tanh-stretched) grid, CD8 viscous terms,
+IBC 108 inflow/isothermal wall, non-reflecting outflow/top, the
+shock-capturing filter every 10 steps, SSP-RK3 - plain FP64.  Synthetic
+seeded inputs.  A step is one RK3 time step (three right-hand sides and
+stages); the metric is grid-point RK-stage updates per second.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `roofline` covers the kernel with the largest share
+of the timed region, timed with HIP events on the library's stream inside the
+timed region; `cpu_baseline` times the CPU oracle (single-threaded C
+restatement of the same routines) on one step of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "opencfd-scu_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic HBM bytes per interior point per launch: each array the kernel
+# must read or write, once (FP64 = 8 B).
+ALGO_BYTES = {
+    "sweep_x": (5 + 3 + 1 + 5) * 8,       # rho,u,v,w,T + 3 cofactors + Ajac; write du(5)
+    "sweep_y": (5 + 3 + 1 + 5 + 5) * 8,   # same + read-modify-write du(5)
+    "sweep_z": (5 + 3 + 1 + 5 + 5) * 8,
+    "visc_grad": (4 + 12) * 8,            # u,v,w,T; write 12 gradients
+    "visc_flux": (12 + 1 + 3 + 10 + 12) * 8,  # gradients, mu, u,v,w, metrics; write 12 Ev
+    "visc_div": (12 + 1 + 4 + 4) * 8,     # 12 Ev, Ajac, du(1..4) read-modify-write
+    "stage": (5 + 5 + 5 + 5 + 6 + 1) * 8,  # fn, f, du; write f, 6 primitives, mu (stages 2-3)
+    "sensor": 2 * 8 + 3,                  # rho, T; write 3 int8 flags
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=512)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_for(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = REPO / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(workload, {}).get(kernel)
+    except Exception:
+        return None
+
+
+def cpu_baseline(case, steps):
+    """The CPU oracle (test infrastructure, used here only as the baseline)
+    on `steps` full time steps of the same grid."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle as O
+    o = case.load(O.Oracle(case.cfg))
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    rate = case.npoints * 3 * steps / dt
+    return {"value": rate, "unit": "grid-point RK-stage updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK3 step(s) of the full {case.cfg.nx}x{case.cfg.ny}x{case.cfg.nz} ramp, "
+                      f"single-threaded C oracle, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+              "(the multi-GPU halo exchange is a design only)", file=sys.stderr)
+        sys.exit(2)
+    from pyocfd import capi, cases
+
+    workload = f"ramp{args.nx}x{args.ny}x{args.nz}"
+    case = cases.ramp(args.nx, args.ny, args.nz, nsteps=args.warmup + args.steps)
+    solver = case.load(capi.Solver(case.cfg, device=0))
+    npts = case.npoints
+    if args.warmup:
+        solver.step(args.warmup)
+    solver.sync()
+    solver.profile(True)
+    solver.profile_reset()
+    t0 = time.perf_counter()
+    solver.step(args.steps)
+    solver.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = solver.last_step_ms()
+    prof = solver.profile_entries()
+    solver.profile(False)
+    secs = ev_ms * 1e-3 if ev_ms > 0 else wall
+    value = npts * 3 * args.steps / secs
+
+    # dominant kernel of the timed region
+    kname, (launches, tot_ms) = max(prof.items(), key=lambda kv: kv[1][1])
+    avg_s = tot_ms * 1e-3 / launches
+    roof = None
+    if kname in ALGO_BYTES:
+        bytes_per_launch = ALGO_BYTES[kname] * npts
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": traffic_for(kname, workload), "algo_bytes_per_launch": bytes_per_launch,
+                "avg_launch_ms": round(avg_s * 1e3, 4)}
+    share = {k: round(v[1] / sum(x[1] for x in prof.values()), 4) for k, v in sorted(prof.items())}
+    flags = solver.get_flags()
+    import numpy as np
+    cnt = np.bincount(np.concatenate([f.ravel() for f in flags]), minlength=4)[1:4]
+    cpu = None
+    if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        cpu = cpu_baseline(case, args.cpu_sample_steps)
+    out = {
+        "metric": "grid-point RK-stage updates/sec on 1 MI355X",
+        "value": value,
+        "unit": "grid-point RK-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded generator: tanh-stretched 24-degree ramp, Blasius-like inflow)",
+        "config": {"workload": workload, "points": npts, "scheme": "hybrid_auto WENO7/UP7 + CD8",
+                   "ibc": 108, "filter_every": 10, "precision": "fp64",
+                   "hybrid_flag_fractions": {"UP7": round(cnt[0] / cnt.sum(), 4),
+                                             "WENO7": round(cnt[1] / cnt.sum(), 4),
+                                             "WENO5": round(cnt[2] / cnt.sum(), 4)}},
+        "wall_s": wall,
+        "kernel_share": share,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,402 @@
+// k_point.hip - pointwise and boundary kernels (gfx950): RK stage fused with
+// cons->prim and viscosity, periodic face fills, IBC 108, the face refresh,
+// the hybrid_auto sensor, the shock-capturing filter and the metric kernels.
+#include "ocfd_dev.h"
+#include "ocfd_kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Periodic face fill, exchange_boundary_{x,y,z}_packed_dev at one rank
+// (src/OCFD_mpi_dev.cu:186-310): ghost[n..n+3] <- interior[0..3],
+// ghost[-4..-1] <- interior[n-4..n-1], other two indices interior only.
+// ---------------------------------------------------------------------------
+__global__ void k_fill(Geo g, double* base, int nf, long long fs, int dir) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;  // position in the 8 x a x b slab
+    const int na = dir == 0 ? g.ny : g.nx;
+    const int nb = dir == 2 ? g.ny : g.nz;
+    const long long tot = 8LL * na * nb;
+    if (t >= tot) return;
+    int l, a, b;
+    if (dir == 0) { l = t % 8; a = (t / 8) % na; b = t / (8 * na); }
+    else { a = t % na; l = (t / na) % 8; b = t / (8 * na); }
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    int dst, src;
+    if (l < 4) { dst = n + l; src = l; } else { dst = l - 8; src = n + l - 8; }
+    long long qd, qs;
+    if (dir == 0) { qd = gidx(g, dst, a, b); qs = gidx(g, src, a, b); }
+    else if (dir == 1) { qd = gidx(g, a, dst, b); qs = gidx(g, a, src, b); }
+    else { qd = gidx(g, a, b, dst); qs = gidx(g, a, b, src); }
+    for (int m = 0; m < nf; m++) base[m * fs + qd] = base[m * fs + qs];
+}
+
+// ---------------------------------------------------------------------------
+// SSP-RK3 stage (OCFD_time_advance_ker1/2/3, src/OCFD_time.cu:44-94) fused
+// with cons->prim + p (cons_to_pri_kernel, src/commen_kernel.cu:77-109) and
+// Sutherland's law (get_Amu_kernal, src/commen_kernel.cu:122-131).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stage(StageArgs A) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= A.g.nx || j >= A.g.ny) return;
+    const long long q = gidx(A.g, i, j, k), fs = A.fs;
+    const double dt = A.dt;
+    double fv[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const long long o = m * fs + q;
+        double r;
+        if (A.krk == 1) {
+            r = A.fn[o] + dt * A.du[o];
+        } else if (A.krk == 2) {
+            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * A.du[o]);
+        } else {
+            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * A.du[o]);
+        }
+        A.f[o] = r;
+        if (A.flap) A.flap[o] = r;
+        fv[m] = r;
+    }
+    const double d1 = fv[0];
+    const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
+    const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
+    A.d[q] = d1; A.u[q] = u1; A.v[q] = v1; A.w[q] = w1; A.T[q] = T2;
+    A.P[q] = T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama);
+    A.amu[q] = A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2);
+}
+
+// ---------------------------------------------------------------------------
+// IBC 108, bc_user_Compression_conner (src/OCFD_boundary_compression_conner.cu
+// 27-179), launched in the reference's order: inlet, top, wall velocity, wall
+// temperature/density with mirrored ghosts.  Interior indices; i<0 is ghost.
+// ---------------------------------------------------------------------------
+__global__ void k_bc_inlet(BCArgs A) {  // do_ub1_inlet_kernel
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (j >= A.g.ny) return;
+    const int ny = A.g.ny;
+    for (int i = -4; i <= 0; i++) {
+        const long long q = gidx(A.g, i, j, k);
+        A.d[q] = A.ub1[j]; A.u[q] = A.ub1[j + ny]; A.v[q] = A.ub1[j + 2 * ny]; A.w[q] = 0.; A.T[q] = A.ub1[j + 3 * ny];
+    }
+}
+__global__ void k_bc_top(BCArgs A) {  // do_ub1_top_kernel
+    const int i = A.x_begin + blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= A.g.nx) return;
+    const int ny = A.g.ny;
+    const long long q = gidx(A.g, i, ny - 1, k);
+    A.d[q] = A.ub1[ny - 1]; A.u[q] = A.ub1[ny + ny - 1]; A.v[q] = A.ub1[2 * ny + ny - 1]; A.w[q] = 0.;
+    A.T[q] = A.ub1[3 * ny + ny - 1];
+}
+__global__ void k_bc_wall_vel(BCArgs A) {  // do_wall_dist_kernel
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= A.g.nx) return;
+    const long long q = gidx(A.g, i, 0, k);
+    A.u[q] = 0.;
+    A.v[q] = A.epsl * A.fx[i] * A.gz[k] * A.ht;
+    A.w[q] = 0.;
+}
+__global__ void k_bc_wall_T(BCArgs A) {  // do_wall_Tp1_kernel / do_wall_Tp2_kernel
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (i >= A.g.nx) return;
+    const Geo& g = A.g;
+    const long long q0 = gidx(g, i, 0, k), q1 = gidx(g, i, 1, k), q2 = gidx(g, i, 2, k), q3 = gidx(g, i, 3, k);
+    if (A.tw > 0) {
+        const double pw = (18. * A.d[q1] * A.T[q1] - 9. * A.d[q2] * A.T[q2] + 2. * A.d[q3] * A.T[q3]) / 11.;
+        A.T[q0] = A.tw;
+        A.d[q0] = pw / A.T[q0];
+        for (int l = 0; l < 4; l++) {  // ghost j = l-4 mirrors interior j = 4-l
+            const long long qg = gidx(g, i, l - 4, k), qm = gidx(g, i, 4 - l, k);
+            A.d[qg] = A.d[qm];
+            A.u[qg] = -A.u[qm];
+            A.v[qg] = -A.v[qm];
+            A.w[qg] = -A.w[qm];
+            A.T[qg] = A.T[qm];
+        }
+    } else {
+        A.T[q0] = (18. * A.T[q1] - 9. * A.T[q2] + 2. * A.T[q3]) / 11.;
+        const double pw = (18. * A.d[q1] * A.T[q1] - 9. * A.d[q2] * A.T[q2] + 2. * A.d[q3] * A.T[q3]) / 11.;
+        A.d[q0] = pw / A.T[q0];
+    }
+}
+// Viscosity on the faces IBC 108 may change (get_Amu runs after OCFD_bc,
+// src/OCFD_NS_Solver.cu:83-87): x=0, y=0, y=ny-1.
+__global__ void k_bc_amu(BCArgs A) {
+    const Geo& g = A.g;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    int i, j;
+    if (t < g.ny) { i = 0; j = t; }
+    else if (t < g.ny + g.nx) { i = t - g.ny; j = 0; }
+    else if (t < g.ny + 2 * g.nx) { i = t - g.ny - g.nx; j = g.ny - 1; }
+    else return;
+    const long long q = gidx(g, i, j, k);
+    const double T = A.T[q];
+    A.amu[q] = A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T);
+}
+
+// Face refresh of OCFD_bc (src/OCFD_boundary.cu:40-91): prim->cons
+// (pri_to_cons_kernel, src/commen_kernel.cu:32-64) on all six faces.
+__device__ __forceinline__ void p2c(const BCArgs& A, long long q) {
+    const double d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
+    A.f[q] = d;
+    A.f[A.fs + q] = u * d;
+    A.f[2 * A.fs + q] = v * d;
+    A.f[3 * A.fs + q] = w * d;
+    A.f[4 * A.fs + q] = d * ((u * u + v * v + w * w) * 0.5 + A.c.cv * T);
+}
+__global__ void k_face_refresh(BCArgs A, int dir) {
+    const Geo& g = A.g;
+    const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * na * nb) return;
+    const int side = t / (na * nb), r = t % (na * nb), a = r % na, b = r / na;
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    const int s = side ? n - 1 : 0;
+    long long q;
+    if (dir == 0) q = gidx(g, s, a, b);
+    else if (dir == 1) q = gidx(g, a, s, b);
+    else q = gidx(g, a, b, s);
+    p2c(A, q);
+}
+
+// ---------------------------------------------------------------------------
+// hybrid_auto style 2 (Comput_P + Comput_Scheme_point_{x,y,z}_Jameson_kernel,
+// src/OCFD_Schemes_hybrid_auto.cu:48-68, 611-703): p = p00*rho*T on the fly,
+// dp = sum over directions |p- - 2p + p+| / (p- + 2p + p+), kp = 1 + (dp>i1) +
+// (dp>i2).  Node (i,j,k) with i,j,k >= -1 serves the x flags when j,k >= 0,
+// the y flags when i,k >= 0 and the z flags when i,j >= 0.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const double* __restrict__ d,
+                                                const double* __restrict__ T, int8_t* fx, int8_t* fy, int8_t* fz) {
+    const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1, k = blockIdx.z - 1;
+    if (i >= g.nx || j >= g.ny) return;
+    const long long q = gidx(g, i, j, k);
+    auto P = [&](long long o) { return c.p00 * d[o] * T[o]; };
+    const double pc = P(q);
+    const long long sx = 1, sy = g.ldx, sz = g.sz;
+    double pm = P(q - sx), pp = P(q + sx);
+    double dp0 = fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
+    pm = P(q - sy); pp = P(q + sy);
+    dp0 += fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
+    pm = P(q - sz); pp = P(q + sz);
+    dp0 += fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
+    int kp = 1;
+    if (dp0 > c.hy_i1) kp += 1;
+    if (dp0 > c.hy_i2) kp += 1;
+    if (j >= 0 && k >= 0) fx[q] = (int8_t)kp;
+    if (i >= 0 && k >= 0) fy[q] = (int8_t)kp;
+    if (i >= 0 && j >= 0) fz[q] = (int8_t)kp;
+}
+
+// ---------------------------------------------------------------------------
+// Shock-capturing filter (filter_{x,y}3d_shock_kernel, src/OCFD_filtering.cu:
+// 366-416, 451-495) for one direction over a box of output points.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double filt_sigma(const double* __restrict__ P, long long L, long long st, double rth) {
+    const double p_m2 = P[L - 2 * st], p_m1 = P[L - st], p_00 = P[L], p_p1 = P[L + st], p_p2 = P[L + 2 * st];
+    const double dp0 = 0.25 * (-p_p1 + 2.0 * p_00 - p_m1);
+    const double dp1 = 0.25 * (-p_p2 + 2.0 * p_p1 - p_00);
+    const double dp2 = 0.25 * (-p_00 + 2.0 * p_m1 - p_m2);
+    double ri = 0.5 * ((dp0 - dp1) * (dp0 - dp1) + (dp0 - dp2) * (dp0 - dp2)) / (p_00 * p_00) + 1e-16;
+    ri = 1.0 - rth / ri;
+    return 0.5 * (ri + fabs(ri));
+}
+__global__ void __launch_bounds__(256) k_filter(FilterArgs A) {
+    const int i = A.lo[0] + blockIdx.x * 64 + threadIdx.x, j = A.lo[1] + blockIdx.y * 4 + threadIdx.y;
+    const int k = A.lo[2] + blockIdx.z;
+    if (i >= A.hi[0] || j >= A.hi[1] || k >= A.hi[2]) return;
+    const Geo& g = A.g;
+    const long long L = gidx(g, i, j, k);
+    const long long st = A.dir == 0 ? 1 : A.dir == 1 ? (long long)g.ldx : g.sz;
+    const double hm = filt_sigma(A.P, L - st, st, A.rth);
+    const double h0 = filt_sigma(A.P, L, st, A.rth);
+    const double hp = filt_sigma(A.P, L + st, st, A.rth);
+    const double Sc1 = 0.5 * (h0 + hp) * A.s0;
+    const double Sc2 = 0.5 * (h0 + hm) * A.s0;
+    const double c1 = -0.2103830, c2 = 0.0396170;
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const double* F = A.flap + (long long)m * A.fs;
+        A.f[(long long)m * A.fs + L] =
+            F[L] - (Sc1 * (c1 * (F[L + st] - F[L]) + c2 * (F[L + 2 * st] - F[L - st])) -
+                    Sc2 * (c1 * (F[L] - F[L - st]) + c2 * (F[L + st] - F[L - 2 * st])));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Metrics (init): OCFD_d?0_CD6_kernel (src/OCFD_Schemes.cu:1236-1300) with the
+// OCFD_D?0_bound_kernel_m/p closures (src/OCFD_bound_Scheme.cu:189-349), then
+// comput_Jac3d_kernal (src/OCFD_Comput_Jacobian3d.cu:250-391).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_metric_deriv(Geo g, const double* __restrict__ a, int dir, int bound,
+                                                      double* out) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const long long q = gidx(g, i, j, k);
+    const long long st = dir == 0 ? 1 : dir == 1 ? (long long)g.ldx : g.sz;
+    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    const int p = dir == 0 ? i : dir == 1 ? j : k;
+    const double h = dir == 0 ? g.hx : dir == 1 ? g.hy : g.hz;
+    auto F = [&](int o) { return a[q + (long long)o * st]; };
+    double r = (45.0 * (F(1) - F(-1)) - 9.0 * (F(2) - F(-2)) + (F(3) - F(-3))) / (60.0 * h);
+    if (bound && n >= 4) {
+        // left closures written relative to node 0 (F(o) is node p+o)
+        if (p == 0) r = (F(1) - F(0)) / h;
+        else if (p == 1) r = (F(1) - F(-1)) * 0.5 / h;
+        else if (p == 2) r = (F(-2) - 8.0 * F(-1) + 8.0 * F(1) - F(2)) / (12.0 * h);
+        else if (p == 3) r = (F(3) - F(-3) - 9.0 * (F(2) - F(-2)) + 45.0 * (F(1) - F(-1))) / (60.0 * h);
+        // right closures (run after the left ones in the reference)
+        if (p == n - 1) r = dir == 1 ? -(F(-1) - F(0)) / h : (F(0) - F(-1)) / h;
+        else if (p == n - 2) r = dir == 1 ? -(F(-1) - F(1)) * 0.5 / h : (F(1) - F(-1)) * 0.5 / h;
+        else if (p == n - 3)
+            r = dir == 1 ? -(F(2) - 8.0 * F(1) + 8.0 * F(-1) - F(-2)) / (12.0 * h)
+                         : (F(-2) - 8.0 * F(-1) + 8.0 * F(1) - F(2)) / (12.0 * h);
+        else if (p == n - 4)
+            r = dir == 1 ? -(F(-3) - F(3) - 9.0 * (F(-2) - F(2)) + 45.0 * (F(-1) - F(1))) / (60.0 * h)
+                         : (F(3) - F(-3) - 9.0 * (F(2) - F(-2)) + 45.0 * (F(1) - F(-1))) / (60.0 * h);
+    }
+    out[q] = r;
+}
+
+struct MetricOut { double* M[M_COUNT]; };
+
+__global__ void __launch_bounds__(256) k_cofactors(Geo g, const double* __restrict__ D, long long fs,
+                                                   MetricOut O) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const long long q = gidx(g, i, j, k);
+    // D: xi xj xk yi yj yk zi zj zk
+    const double xi1 = D[q], xj1 = D[fs + q], xk1 = D[2 * fs + q];
+    const double yi1 = D[3 * fs + q], yj1 = D[4 * fs + q], yk1 = D[5 * fs + q];
+    const double zi1 = D[6 * fs + q], zj1 = D[7 * fs + q], zk1 = D[8 * fs + q];
+    double v[M_COUNT];
+    v[M_JAC] = 1.0 / (xi1 * yj1 * zk1 + yi1 * zj1 * xk1 + zi1 * xj1 * yk1 - zi1 * yj1 * xk1 - yi1 * xj1 * zk1 -
+                      xi1 * zj1 * yk1);
+    v[M_KX] = (yj1 * zk1 - zj1 * yk1);
+    v[M_KY] = (zj1 * xk1 - xj1 * zk1);
+    v[M_KZ] = (xj1 * yk1 - yj1 * xk1);
+    v[M_IX] = (yk1 * zi1 - zk1 * yi1);
+    v[M_IY] = (zk1 * xi1 - xk1 * zi1);
+    v[M_IZ] = (xk1 * yi1 - yk1 * xi1);
+    v[M_SX] = (yi1 * zj1 - zi1 * yj1);
+    v[M_SY] = (zi1 * xj1 - xi1 * zj1);
+    v[M_SZ] = (xi1 * yj1 - yi1 * xj1);
+#pragma unroll
+    for (int m = 0; m < M_COUNT; m++) {
+        double* a = O.M[m];
+        a[q] = v[m];
+        // zeroth-order ghost copies on the six faces
+        if (i == 0) for (int l = 1; l <= 4; l++) a[gidx(g, -l, j, k)] = v[m];
+        if (i == g.nx - 1) for (int l = 1; l <= 4; l++) a[gidx(g, i + l, j, k)] = v[m];
+        if (j == 0) for (int l = 1; l <= 4; l++) a[gidx(g, i, -l, k)] = v[m];
+        if (j == g.ny - 1) for (int l = 1; l <= 4; l++) a[gidx(g, i, j + l, k)] = v[m];
+        if (k == 0) for (int l = 1; l <= 4; l++) a[gidx(g, i, j, -l)] = v[m];
+        if (k == g.nz - 1) for (int l = 1; l <= 4; l++) a[gidx(g, i, j, k + l)] = v[m];
+    }
+}
+
+// Whole-interior prim->cons (src/OCFD_init.cu:99-110) and viscosity
+// (get_Amu, src/commen_kernel.cu:133-140), used at initialisation.
+__global__ void __launch_bounds__(256) k_p2c_all(BCArgs A) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= A.g.nx || j >= A.g.ny) return;
+    p2c(A, gidx(A.g, i, j, k));
+}
+// cons->prim + p at every interior point (get_duvwT, src/commen_kernel.cu:111-118).
+__global__ void __launch_bounds__(256) k_c2p_all(StageArgs A) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= A.g.nx || j >= A.g.ny) return;
+    const long long q = gidx(A.g, i, j, k), fs = A.fs;
+    const double d1 = A.f[q];
+    const double u1 = A.f[fs + q] / d1, v1 = A.f[2 * fs + q] / d1, w1 = A.f[3 * fs + q] / d1;
+    const double T2 = (A.f[4 * fs + q] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
+    A.d[q] = d1; A.u[q] = u1; A.v[q] = v1; A.w[q] = w1; A.T[q] = T2;
+    A.P[q] = T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama);
+}
+__global__ void __launch_bounds__(256) k_amu_all(BCArgs A) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= A.g.nx || j >= A.g.ny) return;
+    const long long q = gidx(A.g, i, j, k);
+    const double T = A.T[q];
+    A.amu[q] = A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T);
+}
+
+__global__ void k_set(double* p, long long n, double v) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+dim3 pgrid(const Geo& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
+
+}  // namespace
+
+void launch_fill(const Geo& g, double* base, int nf, long long fs, int dir, hipStream_t st) {
+    const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
+    const long long tot = 8LL * na * nb;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, g, base, nf, fs, dir);
+}
+
+void launch_stage(const StageArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage, pgrid(a.g), dim3(64, 4), 0, st, a);
+}
+
+void launch_bc108(const BCArgs& a, hipStream_t st) {
+    const Geo& g = a.g;
+    hipLaunchKernelGGL(k_bc_inlet, dim3((g.ny + 63) / 64, g.nz), dim3(64), 0, st, a);
+    if (g.nx - a.x_begin > 0)
+        hipLaunchKernelGGL(k_bc_top, dim3((g.nx - a.x_begin + 63) / 64, g.nz), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc_wall_vel, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
+    if (a.wall_not_normal == 0) hipLaunchKernelGGL(k_bc_wall_T, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
+}
+
+void launch_face_refresh(const BCArgs& a, hipStream_t st) {
+    const Geo& g = a.g;
+    if (a.amu) {
+        const int nt = g.ny + 2 * g.nx;
+        hipLaunchKernelGGL(k_bc_amu, dim3((nt + 63) / 64, g.nz), dim3(64), 0, st, a);
+    }
+    for (int dir = 0; dir < 3; dir++) {
+        const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
+        const long long tot = 2LL * na * nb;
+        hipLaunchKernelGGL(k_face_refresh, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, dir);
+    }
+}
+
+void launch_sensor(const Geo& g, const Consts& c, const double* d, const double* T, int8_t* fx, int8_t* fy,
+                   int8_t* fz, hipStream_t st) {
+    dim3 grid((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    hipLaunchKernelGGL(k_sensor, grid, dim3(64, 4), 0, st, g, c, d, T, fx, fy, fz);
+}
+
+void launch_filter(const FilterArgs& a, hipStream_t st) {
+    const int ex = a.hi[0] - a.lo[0], ey = a.hi[1] - a.lo[1], ez = a.hi[2] - a.lo[2];
+    if (ex <= 0 || ey <= 0 || ez <= 0) return;
+    hipLaunchKernelGGL(k_filter, dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(64, 4), 0, st, a);
+}
+
+void launch_metric_deriv(const Geo& g, const double* coord, int dir, int bound, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_metric_deriv, pgrid(g), dim3(64, 4), 0, st, g, coord, dir, bound, out);
+}
+
+void launch_cofactors(const Geo& g, const double* D9, long long fs, double* const* M, hipStream_t st) {
+    MetricOut o;
+    for (int m = 0; m < M_COUNT; m++) o.M[m] = M[m];
+    hipLaunchKernelGGL(k_cofactors, pgrid(g), dim3(64, 4), 0, st, g, D9, fs, o);
+}
+
+void launch_p2c_all(const BCArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_p2c_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+}
+
+void launch_c2p_all(const StageArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_c2p_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+}
+
+void launch_amu_all(const BCArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_amu_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+}
+
+void launch_set(const Geo& g, double* base, long long n, double value, hipStream_t st) {
+    (void)g;
+    hipLaunchKernelGGL(k_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, base, n, value);
+}

@@ -174,6 +174,41 @@ def tgv(n=64, ma=0.1, re=1600.0, noise=1e-4, seed=SEED, cfl=0.5, scheme="WENO7",
 
 
 # --------------------------------------------------------------------------
+# Curvilinear periodic box (parity workload: every metric term non-zero)
+# --------------------------------------------------------------------------
+def wavy_box(nx=20, ny=16, nz=12, ma=0.5, re=500.0, amp=0.04, seed=SEED, cfl=0.3, scheme="WENO7",
+             scheme_vis="CD6", periodic=(1, 1, 1), precision=capi.PREC_FP64) -> Case:
+    """Periodic box whose grid lines are smooth periodic waves (so all nine
+    cofactors and the Jacobian vary) with a smooth seeded flow.  Metrics come
+    from the mesh (JAC_BOUND = 1 1 1), as in OCFD3d-Mesh.dat runs."""
+    gamma = 1.4
+    rng = np.random.default_rng(seed)
+    i = np.arange(nx) / nx
+    j = np.arange(ny) / ny
+    k = np.arange(nz) / nz
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    tp = 2 * np.pi
+    x = I + amp * np.sin(tp * J) * np.cos(tp * K)
+    y = 0.8 * J + amp * np.sin(tp * I + 0.3) * np.cos(tp * K)
+    z = 0.6 * K + amp * np.sin(tp * I) * np.sin(tp * J + 0.7)
+    ph = rng.uniform(0, tp, 12)
+    d = 1.0 + 0.1 * np.sin(tp * I + ph[0]) * np.cos(tp * J + ph[1]) * np.cos(tp * K + ph[2])
+    u = 0.3 + 0.2 * np.sin(tp * I + ph[3]) * np.cos(tp * J + ph[4]) * np.sin(tp * K + ph[5])
+    v = -0.1 + 0.2 * np.cos(tp * I + ph[6]) * np.sin(tp * J + ph[7])
+    w = 0.05 + 0.15 * np.sin(tp * J + ph[8]) * np.cos(tp * K + ph[9])
+    T = 1.0 + 0.1 * np.cos(tp * I + ph[10]) * np.sin(tp * K + ph[11])
+    h = min(1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    c = np.sqrt(T) / ma
+    dt = float(cfl * h / (np.abs(u) + np.abs(v) + np.abs(w) + 3 * c).max() * 0.5)
+    cfg = capi.make_config(
+        nx=nx, ny=ny, nz=nz, iperiodic=periodic, jac_bound=(1, 1, 1), dif_bound=(0,) * 6, non_ref=(0,) * 6,
+        scheme_invis=scheme, scheme_vis=scheme_vis, hy_style=2, hy_dp_intv=(0.01, 0.2), re=re, ama=ma,
+        gamma=gamma, pr=0.72, t_ref=288.15, epsl_sw=0.0, dt=dt, end_time=end_time_for(10, dt), ibc=0,
+        precision=precision)
+    return Case("wavy%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z))
+
+
+# --------------------------------------------------------------------------
 # Compression ramp (2-D extruded or swept)
 # --------------------------------------------------------------------------
 def _tanh_stretch(ny, first):
