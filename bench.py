@@ -51,7 +51,22 @@ def parse():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB five times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": 5}))
 
 
 def traffic_for(kernel, workload):
@@ -83,6 +98,9 @@ def cpu_baseline(case, steps):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
               "(the multi-GPU halo exchange is a design only)", file=sys.stderr)
