@@ -220,12 +220,13 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) {
             const double ajac = A.jac[q];
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
+            const double rh = 1.0 / A.h;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 const double pr = sh[(m * NF + n + 1) * NA + a], pl = sh[(m * NF + n) * NA + a];
                 const double mr = sh[((5 + m) * NF + n + 1) * NA + a], ml = sh[((5 + m) * NF + n) * NA + a];
-                const double tP = zp ? 0.0 : -ajac * (pr - pl) / A.h;
-                const double tM = zm ? 0.0 : -ajac * (mr - ml) / A.h;
+                const double tP = zp ? 0.0 : -ajac * (pr - pl) * rh;
+                const double tM = zm ? 0.0 : -ajac * (mr - ml) * rh;
                 double* dv = A.du + (long long)m * A.fs;
                 double acc = FIRST ? 0.0 : dv[q];
                 acc = acc + tP;
@@ -237,16 +238,111 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgs A) {
     (void)sstride;
 }
 
+// y and z sweeps: one thread per (x, across) column marching along the sweep
+// direction over a chunk of L output nodes.  The f+/f- of the 8 nodes a face
+// needs (j-3..j+4) live in a register queue: each node is split once per chunk
+// (plus the 8 warm-up nodes), rows of 64 x-consecutive nodes are read by one
+// wave (512 contiguous bytes), and there is no LDS and no barrier.  The next
+// node's inputs are fetched one iteration ahead.
+template <int DIR, int SCH, int L>
+__global__ void __launch_bounds__(256) k_march(SweepArgs A) {
+    static_assert(DIR == 1 || DIR == 2, "x is swept by k_sweep");
+    const Geo& g = A.g;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int a = blockIdx.y * 4 + threadIdx.y;  // z for the y sweep, y for the z sweep
+    const int n_dir = DIR == 1 ? g.ny : g.nz;
+    const int n_acr = DIR == 1 ? g.nz : g.ny;
+    const int s0 = blockIdx.z * L;
+    const int s1 = min(s0 + L, n_dir);
+    const bool own = i < g.nx && a < n_acr;
+    const int ic = min(i, g.nx - 1), acl = min(a, n_acr - 1);
+    const long long st = DIR == 1 ? (long long)g.ldx : g.sz;
+    const long long base = DIR == 1 ? gidx(g, ic, 0, acl) : gidx(g, ic, acl, 0);
+    const double* __restrict__ pd = A.d;
+    const double* __restrict__ pu = A.u;
+    const double* __restrict__ pv = A.v;
+    const double* __restrict__ pw = A.w;
+    const double* __restrict__ pT = A.T;
+    const double* __restrict__ p1 = A.m1;
+    const double* __restrict__ p2 = A.m2;
+    const double* __restrict__ p3 = A.m3;
+    const double rh = 1.0 / A.h;
+
+    double qp[5][8], qm[5][8];
+    double in[8];
+    auto fetch = [&](int s) {
+        const long long q = base + (long long)s * st;
+        in[0] = pd[q]; in[1] = pu[q]; in[2] = pv[q]; in[3] = pw[q]; in[4] = pT[q];
+        in[5] = p1[q]; in[6] = p2[q]; in[7] = p3[q];
+    };
+    auto split_into = [&](int slot) {
+        const double cc = sqrt(in[4]) / A.c.ama;  // sound_speed_kernel, src/commen_kernel.cu:144-153
+        double fp[5], fm[5];
+        sw_split<double>(in[0] * A.c.tmp0, in[1], in[2], in[3], cc, in[5], in[6], in[7], A.c, fp, fm);
+#pragma unroll
+        for (int m = 0; m < 5; m++) { qp[m][slot] = fp[m]; qm[m][slot] = fm[m]; }
+    };
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+        fetch(s0 - 4 + t);
+        split_into(t);
+    }
+    fetch(s0 + 3);
+    double pp[5] = {0, 0, 0, 0, 0}, pm[5] = {0, 0, 0, 0, 0};
+    for (int j = s0 - 1; j < s1; j++) {
+        split_into(7);
+        if (j + 5 <= n_dir + 3) fetch(j + 5);
+        int hy = 0;
+        if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[base + (long long)j * st];
+        double hp[5], hm[5];
+#pragma unroll
+        for (int m = 0; m < 5; m++)
+            face_pm<double, SCH>(qp[m], qm[m], j + 1, j - n_dir, A.bl, A.br, hy, A.c.tv_lim, A.c.tv_max, hp[m],
+                                 hm[m]);
+        if (j >= s0 && own) {
+            const long long q = base + (long long)j * st;
+            const double ajac = A.jac[q];
+            const bool zp = A.nrp && j == 0;
+            const bool zm = A.nrm && j == n_dir - 1;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const double tP = zp ? 0.0 : -ajac * (hp[m] - pp[m]) * rh;
+                const double tM = zm ? 0.0 : -ajac * (hm[m] - pm[m]) * rh;
+                double* dv = A.du + (long long)m * A.fs;
+                double acc = dv[q];
+                acc = acc + tP;
+                acc = acc + tM;
+                dv[q] = acc;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            pp[m] = hp[m];
+            pm[m] = hm[m];
+#pragma unroll
+            for (int t = 0; t < 7; t++) { qp[m][t] = qp[m][t + 1]; qm[m][t] = qm[m][t + 1]; }
+        }
+    }
+}
+
+#ifndef OCFD_LMARCH
+#define OCFD_LMARCH 32  // output nodes per marching chunk (y and z sweeps)
+#endif
+constexpr int LMARCH = OCFD_LMARCH;
+
 template <int DIR, int SCH>
 void launch_dir(const SweepArgs& a, bool first, hipStream_t st) {
-    constexpr int NS = Tile<DIR>::NS, NA = Tile<DIR>::NA;
     const Geo& g = a.g;
-    dim3 grid;
-    if (DIR == 0) grid = dim3((g.nx + NS - 1) / NS, (g.ny + NA - 1) / NA, g.nz);
-    else if (DIR == 1) grid = dim3((g.nx + NA - 1) / NA, (g.ny + NS - 1) / NS, g.nz);
-    else grid = dim3((g.nx + NA - 1) / NA, (g.nz + NS - 1) / NS, g.ny);
-    if (first) hipLaunchKernelGGL((k_sweep<DIR, SCH, true>), grid, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_sweep<DIR, SCH, false>), grid, dim3(256), 0, st, a);
+    if constexpr (DIR == 0) {
+        constexpr int NS = Tile<0>::NS, NA = Tile<0>::NA;
+        dim3 grid((g.nx + NS - 1) / NS, (g.ny + NA - 1) / NA, g.nz);
+        if (first) hipLaunchKernelGGL((k_sweep<0, SCH, true>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_sweep<0, SCH, false>), grid, dim3(256), 0, st, a);
+    } else {
+        const int n_dir = DIR == 1 ? g.ny : g.nz, n_acr = DIR == 1 ? g.nz : g.ny;
+        dim3 grid((g.nx + 63) / 64, (n_acr + 3) / 4, (n_dir + LMARCH - 1) / LMARCH);
+        hipLaunchKernelGGL((k_march<DIR, SCH, LMARCH>), grid, dim3(64, 4), 0, st, a);
+    }
 }
 
 template <int SCH>

@@ -96,9 +96,10 @@ int timed(ocfd_ctx* x, const char* name, F&& fn) {
         fn();
     } else {
         hipEvent_t a = get_event(x), b = get_event(x);
-        hipEventRecord(a, x->st);
+        if (!a || !b) return fail(OCFD_ERR_HIP, std::string(name) + ": hipEventCreate failed");
+        HIPCK(hipEventRecord(a, x->st));
         fn();
-        hipEventRecord(b, x->st);
+        HIPCK(hipEventRecord(b, x->st));
         x->recs.push_back({name, a, b});
     }
     hipError_t e = hipGetLastError();
@@ -111,7 +112,7 @@ int flush_prof(ocfd_ctx* x) {
     HIPCK(hipStreamSynchronize(x->st));
     for (auto& r : x->recs) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, r.a, r.b);
+        HIPCK(hipEventElapsedTime(&ms, r.a, r.b));
         bool found = false;
         for (auto& e : x->acc)
             if (e.first == r.name) {
@@ -158,8 +159,11 @@ int8_t* alloc_bytes(ocfd_ctx* x, size_t n, int& rc) {
         rc = fail(OCFD_ERR_HIP, "hipMalloc failed");
         return nullptr;
     }
-    hipMemsetAsync(p, 0, n + 256, x->st);
     x->allocs.push_back(p);
+    if (hipMemsetAsync(p, 0, n + 256, x->st) != hipSuccess) {
+        rc = fail(OCFD_ERR_HIP, "hipMemsetAsync failed");
+        return nullptr;
+    }
     return (int8_t*)p + 96;
 }
 
@@ -270,9 +274,8 @@ int rhs(ocfd_ctx* x, int krk) {
     for (int q = 0; q < 6; q++) va.bnd[q] = C.dif_bound[q];
     va.u = x->u; va.v = x->v; va.w = x->w; va.T = x->T; va.amu = x->amu;
     for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-    va.grad = x->grad; va.ev = x->ev; va.du = x->du; va.fs = x->fs;
-    TRY(timed(x, "visc_grad", [&] { launch_visc_grad(va, x->st); }));
-    TRY(timed(x, "visc_flux", [&] { launch_visc_flux(va, x->st); }));
+    va.ev = x->ev; va.du = x->du; va.fs = x->fs;
+    TRY(timed(x, "visc_flux", [&] { launch_visc_gf(va, x->st); }));
     for (int dir = 0; dir < 3; dir++)
         if (C.iperiodic[dir])
             TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, x->ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
@@ -526,9 +529,8 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
         p[6] = std::max(p[6], 0); p[7] = std::min(p[7], C.ny);
         p[8] = std::max(p[8], 0); p[9] = std::min(p[9], C.nz);
     }
-    hipEventCreate(&x->step_a);
-    hipEventCreate(&x->step_b);
-    if (hipStreamSynchronize(x->st) != hipSuccess) {
+    if (hipEventCreate(&x->step_a) != hipSuccess || hipEventCreate(&x->step_b) != hipSuccess ||
+        hipStreamSynchronize(x->st) != hipSuccess) {
         ocfd_destroy(x);
         return fail(OCFD_ERR_HIP, "initial fill failed");
     }
@@ -538,13 +540,14 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
 
 int ocfd_destroy(ocfd_ctx* x) {
     if (!x) return 0;
-    if (x->st) hipStreamSynchronize(x->st);
-    for (auto& r : x->recs) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
-    for (auto e : x->pool) hipEventDestroy(e);
-    if (x->step_a) hipEventDestroy(x->step_a);
-    if (x->step_b) hipEventDestroy(x->step_b);
-    for (void* p : x->allocs) hipFree(p);
-    if (x->st) hipStreamDestroy(x->st);
+    // teardown: errors are not actionable here and are ignored
+    if (x->st) (void)hipStreamSynchronize(x->st);
+    for (auto& r : x->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : x->pool) (void)hipEventDestroy(e);
+    if (x->step_a) (void)hipEventDestroy(x->step_a);
+    if (x->step_b) (void)hipEventDestroy(x->step_b);
+    for (void* p : x->allocs) (void)hipFree(p);
+    if (x->st) (void)hipStreamDestroy(x->st);
     delete x;
     return 0;
 }
@@ -649,7 +652,7 @@ int ocfd_step(ocfd_ctx* x, int nsteps) {
     HIPCK(hipEventRecord(x->step_b, x->st));
     HIPCK(hipEventSynchronize(x->step_b));
     float ms = 0.f;
-    hipEventElapsedTime(&ms, x->step_a, x->step_b);
+    HIPCK(hipEventElapsedTime(&ms, x->step_a, x->step_b));
     x->last_step_ms = ms;
     TRY(flush_prof(x));
     return 0;

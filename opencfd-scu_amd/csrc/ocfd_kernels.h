@@ -28,13 +28,12 @@ struct ViscArgs {
     int bnd[6];                              // DIF_BOUND
     const double *u, *v, *w, *T, *amu;
     const double *M[M_COUNT];
-    double *grad;                            // 12 fields
     double *ev;                              // 12 fields: Ev1..4 for x, y, z
     double *du;                              // 5 fields
     long long fs;
 };
-void launch_visc_grad(const ViscArgs& a, hipStream_t st);
-void launch_visc_flux(const ViscArgs& a, hipStream_t st);
+// Gradients of u,v,w,T (kept in registers) and the 12 viscous fluxes.
+void launch_visc_gf(const ViscArgs& a, hipStream_t st);
 void launch_visc_div(const ViscArgs& a, hipStream_t st);
 
 // Periodic face fill (exchange_boundary_?_packed_dev at one rank): copy the

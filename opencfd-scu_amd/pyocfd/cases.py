@@ -5,9 +5,12 @@ programs (benchmark_comp/swept-compression-ramp.c, init3d-inlet1d.c); these are
 this project's own generators for the workloads it measures:
 
 * ``tgv``   - 3-D Taylor-Green vortex in a periodic box with identity metrics.
+* ``wavy_box`` - curvilinear periodic box (every metric term non-zero).
 * ``ramp``  - 24-degree compression ramp (optionally swept in z) on a
               tanh-stretched wall-normal grid, IBC 108 inflow/wall, hybrid
               WENO7/UP7 + CD8, the shock-capturing filter.
+* ``shock_turbulence`` - Mach 1.5 normal shock in a Cartesian box with a
+              seeded solenoidal von Karman upstream field, WENO7.
 
 Each generator returns a :class:`Case` with the config, the grid (mesh
 coordinates or the 13 Jacobi arrays), the primitive fields and, for the ramp,
@@ -322,3 +325,83 @@ def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=
         filters=filters, precision=precision)
     return Case("ramp%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z), inlet=inlet,
                 extra={"inlet_y": ycol, "nsteps": nsteps})
+
+
+# --------------------------------------------------------------------------
+# Shock-turbulence interaction (Cartesian)
+# --------------------------------------------------------------------------
+def solenoidal_field(shape, lengths, k0, u_rms, seed=SEED):
+    """Seeded divergence-free random velocity (periodic box, `shape` =
+    (nz, ny, nx), `lengths` = (Lz, Ly, Lx)) with the von Karman energy
+    spectrum E(k) ~ (k/k0)^4 / (1 + (k/k0)^2)^(17/6), k0 in units of the
+    fundamental wavenumber of the y period, scaled to the given u_rms (rms of
+    each component)."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    Lz, Ly, Lx = lengths
+    kz = np.fft.fftfreq(nz, Lz / nz) * 2 * np.pi
+    ky = np.fft.fftfreq(ny, Ly / ny) * 2 * np.pi
+    kx = np.fft.rfftfreq(nx, Lx / nx) * 2 * np.pi
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    k2 = KX * KX + KY * KY + KZ * KZ
+    kk = np.sqrt(k2)
+    kref = k0 * 2 * np.pi / Ly
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = (kk / kref) ** 4 / (1.0 + (kk / kref) ** 2) ** (17.0 / 6.0)
+        amp = np.where(kk > 0, np.sqrt(E / (4 * np.pi * np.maximum(k2, 1e-300))), 0.0)
+    comps = []
+    for _ in range(3):
+        c = rng.standard_normal(KX.shape) + 1j * rng.standard_normal(KX.shape)
+        comps.append(c * amp)
+    # projection onto divergence-free fields: u - k (k.u) / |k|^2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dot = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / np.where(k2 > 0, k2, 1.0)
+    comps = [comps[0] - KX * dot, comps[1] - KY * dot, comps[2] - KZ * dot]
+    u = [np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) for c in comps]
+    rms = math.sqrt(sum(float((a * a).mean()) for a in u) / 3.0)
+    return [a * (u_rms / rms) for a in u]
+
+
+def shock_turbulence(nx=512, ny=256, nz=256, ma=1.5, re=2000.0, x_shock=0.25, k0=4.0, u_rms=0.1,
+                     seed=SEED, cfl=0.5, scheme="WENO7", nsteps=500, precision=capi.PREC_FP64) -> Case:
+    """Normal shock at Mach `ma` at x = x_shock*Lx in a Cartesian box
+    (Lx = 4*pi, Ly = Lz = 2*pi, y and z periodic); upstream a seeded
+    solenoidal von Karman velocity field with constant density and pressure
+    (rho = 1, T = 1, mean u = 1); downstream the Rankine-Hugoniot state.  The
+    jump is smoothed over two cells and the fluctuations are tapered off over
+    a few cells ahead of the shock."""
+    gamma = 1.4
+    Lx, Ly, Lz = 4 * math.pi, 2 * math.pi, 2 * math.pi
+    xs = np.arange(nx) * (Lx / (nx - 1))
+    ys = np.arange(ny) * (Ly / ny)
+    zs = np.arange(nz) * (Lz / nz)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    dx = Lx / (nx - 1)
+    m2 = ma * ma
+    r21 = (gamma + 1) * m2 / ((gamma - 1) * m2 + 2)          # rho2/rho1
+    p21 = 1 + 2 * gamma / (gamma + 1) * (m2 - 1)              # p2/p1
+    xs0 = x_shock * Lx
+    s = 0.5 * (1 + np.tanh((xs - xs0) / (2 * dx)))            # 0 upstream, 1 downstream
+    d1 = 1.0 + (r21 - 1.0) * s
+    u1 = 1.0 + (1.0 / r21 - 1.0) * s
+    p1 = (1.0 + (p21 - 1.0) * s) / (gamma * m2)
+    T1 = gamma * m2 * p1 / d1
+    fl = solenoidal_field((nz, ny, nx), (Lz, Ly, Lx), k0, u_rms, seed)
+    taper = 0.5 * (1 - np.tanh((xs - (xs0 - 6 * dx)) / (2 * dx)))
+    d = np.broadcast_to(d1, X.shape).copy()
+    u = u1[None, None, :] + fl[0] * taper
+    v = fl[1] * taper
+    w = fl[2] * taper
+    T = np.broadcast_to(T1, X.shape).copy()
+    c = np.sqrt(T) / ma
+    # inviscid spectral radius; in computational units xi = i*h, so
+    # (|u|+c) * (dxi/dx) / h = (|u|+c) / dx per direction
+    lam = (np.abs(u) + c) / dx + (np.abs(v) + c) / (Ly / ny) + (np.abs(w) + c) / (Lz / nz)
+    dt = float(cfl / lam.max())
+    cfg = capi.make_config(
+        nx=nx, ny=ny, nz=nz, iperiodic=(0, 1, 1), jac_bound=(1, 1, 1),
+        dif_bound=(1, 1, 0, 0, 0, 0), non_ref=(0, 0, 0, 0, 0, 0), scheme_invis=scheme, scheme_vis="CD6",
+        hy_style=2, hy_dp_intv=(0.01, 0.2), re=re, ama=ma, gamma=gamma, pr=0.72, t_ref=288.15, epsl_sw=0.0,
+        dt=dt, end_time=end_time_for(nsteps, dt), ibc=0, precision=precision)
+    return Case("shockturb%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(X, Y, Z),
+                extra={"nsteps": nsteps})

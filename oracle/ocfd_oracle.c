@@ -15,6 +15,9 @@
  * that are never written and therefore stay at their initial value, the order
  * in which terms are added into du) are restated in the drivers.
  *
+ * The point loops are parallel over OpenMP threads (each iteration writes
+ * its own outputs, so the result does not depend on the thread count).
+ *
  * Deliberate choice where the reference is undefined: the hybrid kernels set
  * Hyscheme_flag only for variable 0 (src/OCFD_Schemes_hybrid_auto.cu:874,907,
  * 971,1005) and leave it uninitialised for variables 1..4; this restatement
@@ -23,6 +26,7 @@
 #include "ocfd_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1134,6 +1138,7 @@ static void ocfd_bc(orc *o) {
 
 /* get_Amu, src/commen_kernel.cu:122-140 */
 static void get_amu(orc *o) {
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < o->nz; z++)
         for (int y = 0; y < o->ny; y++)
             for (int x = 0; x < o->nx; x++) {
@@ -1146,6 +1151,7 @@ static void get_amu(orc *o) {
 static void cons_to_pri(orc *o) {
     long N = o->nint;
     double g = o->c.gamma, Ma = o->c.ama;
+#pragma omp parallel for schedule(static)
     for (int z = 0; z < o->nz; z++)
         for (int y = 0; y < o->ny; y++)
             for (int x = 0; x < o->nx; x++) {
@@ -1183,6 +1189,7 @@ static void hybrid_sensor(orc *o) {
     const double *P = o->P;
     for (int dir = 0; dir < 3; dir++) {
         int ex = nx + (dir == 0), ey = ny + (dir == 1), ez = nz + (dir == 2);
+#pragma omp parallel for schedule(static)
         for (int z = 0; z < ez; z++)
             for (int y = 0; y < ey; y++)
                 for (int x = 0; x < ex; x++) {
@@ -1209,8 +1216,10 @@ static void hybrid_sensor(orc *o) {
 static void du_invis(orc *o) {
     fill_xyz(o, o->d); fill_xyz(o, o->u); fill_xyz(o, o->v); fill_xyz(o, o->w); fill_xyz(o, o->T);
     double Ma = o->c.ama;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < o->nlap; i++) o->cc[i] = sqrt(o->T[i]) / Ma;
     const int ai[3][3] = {{AKX, AKY, AKZ}, {AIX, AIY, AIZ}, {ASX, ASY, ASZ}};
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < o->nlap; i++) {
         double tmp0d = o->d[i] * o->tmp0;
         for (int dir = 0; dir < 3; dir++) {
@@ -1239,6 +1248,7 @@ static void du_invis(orc *o) {
             const double *F = side == 0 ? o->fp[dir] : o->fm[dir];
             int nonref = o->c.non_ref[2 * dir + side];
             for (int m = 0; m < 5; m++)
+#pragma omp parallel for collapse(2) schedule(static)
                 for (int b = 0; b < nb; b++)
                     for (int a = 0; a < na; a++) {
                         int X = dir == 0 ? 0 : a, Y = dir == 0 ? a : dir == 1 ? 0 : b, Z = dir == 2 ? 0 : b;
@@ -1270,6 +1280,7 @@ static void du_vis(orc *o) {
             double h = dir == 0 ? o->hx : dir == 1 ? o->hy : o->hz;
             int bl = o->c.dif_bound[2 * dir], br = o->c.dif_bound[2 * dir + 1];
             int na = dir == 0 ? ny : nx, nb = dir == 2 ? ny : nz;
+#pragma omp parallel for collapse(2) schedule(static)
             for (int b = 0; b < nb; b++)
                 for (int a = 0; a < na; a++) {
                     int X = dir == 0 ? 0 : a, Y = dir == 0 ? a : dir == 1 ? 0 : b, Z = dir == 2 ? 0 : b;
@@ -1284,6 +1295,7 @@ static void du_vis(orc *o) {
     const int ai[3][3] = {{AKX, AKY, AKZ}, {AIX, AIY, AIZ}, {ASX, ASY, ASZ}};
     for (int dir = 0; dir < 3; dir++) {
         /* vis_flux_ker, src/OCFD_NS_Jacobian3d.cu:239-281 (interior only) */
+#pragma omp parallel for schedule(static)
         for (int z = 0; z < nz; z++)
             for (int y = 0; y < ny; y++)
                 for (int x = 0; x < nx; x++) {
@@ -1336,6 +1348,7 @@ static void du_vis(orc *o) {
         int bl = o->c.dif_bound[2 * dir], br = o->c.dif_bound[2 * dir + 1];
         int na = dir == 0 ? ny : nx, nb = dir == 2 ? ny : nz;
         for (int k = 0; k < 4; k++)
+#pragma omp parallel for collapse(2) schedule(static)
             for (int b = 0; b < nb; b++)
                 for (int a = 0; a < na; a++) {
                     int X = dir == 0 ? 0 : a, Y = dir == 0 ? a : dir == 1 ? 0 : b, Z = dir == 2 ? 0 : b;
@@ -1343,6 +1356,7 @@ static void du_vis(orc *o) {
                     line_central(sv, n, fl, fs, h, bl, br, o->visd[k] + I0(o, X, Y, Z), os);
                 }
         for (int k = 0; k < 4; k++)
+#pragma omp parallel for schedule(static)
             for (int z = 0; z < nz; z++)
                 for (int y = 0; y < ny; y++)
                     for (int x = 0; x < nx; x++) {
@@ -1367,6 +1381,7 @@ static void time_advance(orc *o, int krk) {
     long N = o->nint;
     double dt = o->c.dt;
     for (int m = 0; m < 5; m++)
+#pragma omp parallel for schedule(static)
         for (int z = 0; z < o->nz; z++)
             for (int y = 0; y < o->ny; y++)
                 for (int x = 0; x < o->nx; x++) {
@@ -1467,10 +1482,12 @@ static void orc_filtering(orc *o) {
         int kb = para[m][8] + LAP, ke = para[m][9] + LAP;
         double s0 = o->c.filter_rpara[m][0], rth = o->c.filter_rpara[m][1];
         if (para[m][1] == 1)
+#pragma omp parallel for schedule(static)
             for (int Z = kb; Z < ke; Z++)
                 for (int Y = jb; Y < je; Y++)
                     for (int X = ib; X < ie - 1; X++) filter_point(o, 0, X, Y, Z, s0, rth);
         if (para[m][2] == 1)
+#pragma omp parallel for schedule(static)
             for (int Z = kb; Z < ke; Z++)
                 for (int Y = jb; Y < je - 1; Y++)
                     for (int X = ib; X < ie; X++) filter_point(o, 1, X, Y, Z, s0, rth);
@@ -1478,6 +1495,7 @@ static void orc_filtering(orc *o) {
             int nby = (je - jb + 7) / 8;
             int yend = jb + 6 * nby;
             if (yend > je - 1) yend = je - 1;
+#pragma omp parallel for schedule(static)
             for (int Z = kb; Z < ke; Z++)
                 for (int Y = jb; Y < yend; Y++)
                     for (int X = ib; X < ie; X++) filter_point(o, 1, X, Y, Z, s0, rth);
@@ -1519,6 +1537,8 @@ int orc_get_flow(orc *o, double *d, double *u, double *v, double *w, double *T, 
     if (tt) *tt = o->tt;
     return 0;
 }
+int orc_num_threads(void) { return omp_get_max_threads(); }
+
 int orc_get_state(orc *o, double *f5) { memcpy(f5, o->f, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_du(orc *o, double *du5) { memcpy(du5, o->du, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_pressure(orc *o, double *p) { get_interior(o, o->P, p); return 0; }

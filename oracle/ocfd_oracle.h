@@ -34,6 +34,8 @@ int orc_set_inlet_profile(orc *o, const double *prof4xny);
 int orc_init(orc *o);
 int orc_rhs(orc *o, int krk);
 int orc_step(orc *o, int nsteps);
+/* OpenMP threads the point loops use. */
+int orc_num_threads(void);
 int orc_get_flow(orc *o, double *d, double *u, double *v, double *w,
                  double *T, int *istep, double *tt);
 int orc_get_state(orc *o, double *f5);

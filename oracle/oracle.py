@@ -21,8 +21,8 @@ from pyocfd.capi import OcfdConfig  # noqa: E402  (interface type only)
 
 def build(force: bool = False) -> Path:
     """Compile liboracle.so with the committed Makefile."""
-    src = ORACLE_DIR / "ocfd_oracle.c"
-    if force or not LIB.exists() or LIB.stat().st_mtime < src.stat().st_mtime:
+    srcs = [ORACLE_DIR / n for n in ("ocfd_oracle.c", "ocfd_oracle.h", "Makefile")]
+    if force or not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in srcs):
         subprocess.run(["make", "-C", str(ORACLE_DIR), "liboracle.so"], check=True,
                        stdout=subprocess.DEVNULL)
     return LIB
@@ -50,6 +50,7 @@ def lib() -> C.CDLL:
             "orc_init": (C.c_int, [vp]),
             "orc_rhs": (C.c_int, [vp, C.c_int]),
             "orc_step": (C.c_int, [vp, C.c_int]),
+            "orc_num_threads": (C.c_int, []),
             "orc_get_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
             "orc_get_state": (C.c_int, [vp, dp]),
             "orc_get_du": (C.c_int, [vp, dp]),
