@@ -6,6 +6,7 @@
 // each launching one P or M kernel per variable with an atomicAdd into du,
 // src/OCFD_Schemes.cu:293-381):
 //
+// x (k_sweep, LDS tile):
 //   phase 1  load rho,u,v,w,T and the direction's cofactors for a tile of
 //            nodes (+4 ghost nodes each side along the sweep), compute the
 //            sound speed and the Steger-Warming f+/f- of all 5 variables,
@@ -16,25 +17,19 @@
 //            += the M term, in the reference's order (x P, x M, y P, ...), with
 //            no atomics.
 //
-// Tiles: x sweep 56 outputs x 4 rows (64 nodes per row = one wave), y and z
-// sweeps 16 x-lanes (one 128-byte row segment) x 15 outputs.
+// y and z (k_tile): the same three phases on tiles of x-consecutive columns;
+// k_march (the same steps per node while marching along the sweep, with the
+// split fluxes in a register queue) is the alternative for the float path.
+//
+// S (double or float) is the type of the primitives, the split fluxes, the
+// reconstructions and du; the WENO weight arithmetic and the Jacobian factor
+// with the accumulation into du are always double.
+#include <type_traits>
+
 #include "ocfd_dev.h"
 #include "ocfd_kernels.h"
 
 namespace {
-
-template <int DIR> struct Tile;
-template <> struct Tile<0> { static constexpr int NS = 56, NA = 4; };
-template <> struct Tile<1> { static constexpr int NS = 15, NA = 16; };
-template <> struct Tile<2> { static constexpr int NS = 15, NA = 16; };
-
-// LDS element (v, n, a): node-contiguous for the x sweep (consecutive lanes
-// hold consecutive x), lane-contiguous for y/z sweeps.
-template <int DIR, int NN, int NA>
-__device__ __forceinline__ int lds_at(int v, int n, int a) {
-    if (DIR == 0) return (v * NA + a) * NN + n;
-    return (v * NN + n) * NA + a;
-}
 
 // Steger-Warming split of one node (src/OCFD_split.cu:92-142).
 template <typename R>
@@ -47,9 +42,18 @@ __device__ __forceinline__ void sw_split(R tmp0d, R u, R v, R w, R cc, R A1, R A
     ss = R(1.0) / ss;
     A1 *= ss; A2 *= ss; A3 *= ss;
     const R eps = R(c.epsl_sw);
-    R E1P = (E1 + sqrt(E1 * E1 + eps * eps)) * R(0.50);
-    R E2P = (E2 + sqrt(E2 * E2 + eps * eps)) * R(0.50);
-    R E3P = (E3 + sqrt(E3 * E3 + eps * eps)) * R(0.50);
+    R E1P, E2P, E3P;
+    if (eps == R(0)) {
+        // sqrt(x*x) == |x| in binary floating point with round-to-nearest
+        // (no underflow of x*x): the same values without three square roots
+        E1P = (E1 + fabs(E1)) * R(0.50);
+        E2P = (E2 + fabs(E2)) * R(0.50);
+        E3P = (E3 + fabs(E3)) * R(0.50);
+    } else {
+        E1P = (E1 + sqrt(E1 * E1 + eps * eps)) * R(0.50);
+        E2P = (E2 + sqrt(E2 * E2 + eps * eps)) * R(0.50);
+        E3P = (E3 + sqrt(E3 * E3 + eps * eps)) * R(0.50);
+    }
     R E1M = E1 - E1P, E2M = E2 - E2P, E3M = E3 - E3P;
     R uc1 = u - cc * A1, uc2 = u + cc * A1;
     R vc1 = v - cc * A2, vc2 = v + cc * A2;
@@ -73,7 +77,7 @@ __device__ __forceinline__ void sw_split(R tmp0d, R u, R v, R w, R cc, R A1, R A
 
 // P and M half-node values at face j+1/2 from gp/gm = f+/f- at nodes j-3..j+4.
 // c = j+1, r = j-n (the reference's coords and coords+start-end).
-template <typename R, int SCH>
+template <typename R, typename W, int SCH>
 __device__ __forceinline__ void face_pm(const R* gp, const R* gm, int c, int r, int bl, int br, int hy,
                                         R tvlim, R tvmax, R& hp, R& hm) {
     constexpr int kaP = Ext<SCH, 0>::ka, kbP = Ext<SCH, 0>::kb;
@@ -81,171 +85,273 @@ __device__ __forceinline__ void face_pm(const R* gp, const R* gm, int c, int r, 
     // ---- P (OCFD_bound_scheme_kernel_p, src/OCFD_bound_Scheme.cu:739-819)
     if (bl && c <= -kaP) {
         if (c == 0) hp = gp[4];
-        else if (c == 1) hp = weno5_up_left(gp + 1);
-        else if (c == 2) hp = weno5_up_leftplus(gp + 1);
-        else hp = weno5_up(gp + 1);
+        else if (c == 1) hp = weno5_up_left<R, W>(gp + 1);
+        else if (c == 2) hp = weno5_up_leftplus<R, W>(gp + 1);
+        else hp = weno5_up<R, W>(gp + 1);
     } else if (br && r >= -kbP) {
         if (r == -1) hp = gp[3] + R(0.5) * d_minmod2(gp[3] - gp[2], gp[3] - gp[2]);
-        else if (r == -2) hp = weno5_up_rightplus(gp + 1);
-        else hp = weno5_up(gp + 1);
+        else if (r == -2) hp = weno5_up_rightplus<R, W>(gp + 1);
+        else hp = weno5_up<R, W>(gp + 1);
     } else {
-        if (SCH == OCFD_WENO7) hp = weno7_up(gp);
+        if (SCH == OCFD_WENO7) hp = weno7_up<R, W>(gp);
         else if (SCH == OCFD_UP7) hp = up7_up(gp);
-        else if (SCH == OCFD_WENO5) hp = weno5_up(gp + 1);
+        else if (SCH == OCFD_WENO5) hp = weno5_up<R, W>(gp + 1);
         else if (SCH == OCFD_NND2) hp = nnd2_up(gp + 2);
-        else if (SCH == OCFD_WENO7_SYMBO) hp = symbo_up(gp, 0, tvlim, tvmax);
-        else if (SCH == OCFD_WENO7_SYMBO_LIM) hp = symbo_up(gp, 1, tvlim, tvmax);
+        else if (SCH == OCFD_WENO7_SYMBO) hp = symbo_up<R, W>(gp, 0, tvlim, tvmax);
+        else if (SCH == OCFD_WENO7_SYMBO_LIM) hp = symbo_up<R, W>(gp, 1, tvlim, tvmax);
         else if (SCH == OCFD_OMP6_HR) hp = omp6_up(0, gp);
         else if (SCH == OCFD_OMP6_LD) hp = omp6_up(1, gp);
         else if (SCH == OCFD_OMP6_CD8) hp = omp6_up(2, gp);
         else {  // hybrid_auto, src/OCFD_Schemes_hybrid_auto.cu:911-919
             if (hy == 1) hp = up7_up(gp);
-            else if (hy == 2) hp = weno7_up(gp);
-            else hp = weno5_up(gp + 1);
+            else if (hy == 2) hp = weno7_up<R, W>(gp);
+            else hp = weno5_up<R, W>(gp + 1);
         }
     }
     // ---- M (OCFD_bound_scheme_kernel_m, src/OCFD_bound_Scheme.cu:855-944)
     if (bl && c < -kaM) {
-        if (c == 0) hm = weno5_dn_left(gm + 2);
-        else if (c == 1) hm = weno5_dn_leftplus(gm + 2);
-        else hm = weno5_dn(gm + 2);
+        if (c == 0) hm = weno5_dn_left<R, W>(gm + 2);
+        else if (c == 1) hm = weno5_dn_leftplus<R, W>(gm + 2);
+        else hm = weno5_dn<R, W>(gm + 2);
     } else if (br && r >= -kbM - 1) {
         if (r == -1) hm = gm[3];
         else if (r == -2) hm = gm[4] - R(0.5) * d_minmod2(gm[4] - gm[3], gm[4] - gm[3]);
-        else if (r == -3) hm = weno5_dn_rightplus(gm + 2);
-        else hm = weno5_dn(gm + 2);
+        else if (r == -3) hm = weno5_dn_rightplus<R, W>(gm + 2);
+        else hm = weno5_dn<R, W>(gm + 2);
     } else {
-        if (SCH == OCFD_WENO7) hm = weno7_dn(gm + 1);
+        if (SCH == OCFD_WENO7) hm = weno7_dn<R, W>(gm + 1);
         else if (SCH == OCFD_UP7) hm = up7_dn(gm);
-        else if (SCH == OCFD_WENO5) hm = weno5_dn(gm + 2);
+        else if (SCH == OCFD_WENO5) hm = weno5_dn<R, W>(gm + 2);
         else if (SCH == OCFD_NND2) hm = nnd2_dn(gm + 3);
-        else if (SCH == OCFD_WENO7_SYMBO || SCH == OCFD_WENO7_SYMBO_LIM) hm = symbo_dn(gm);
+        else if (SCH == OCFD_WENO7_SYMBO || SCH == OCFD_WENO7_SYMBO_LIM) hm = symbo_dn<R, W>(gm);
         else if (SCH == OCFD_OMP6_HR) hm = omp6_dn(0, gm);
         else if (SCH == OCFD_OMP6_LD) hm = omp6_dn(1, gm);
         else if (SCH == OCFD_OMP6_CD8) hm = omp6_dn(2, gm);
         else {  // src/OCFD_Schemes_hybrid_auto.cu:1008-1016
             if (hy == 1) hm = up7_dn(gm);
-            else if (hy == 2) hm = weno7_dn(gm + 1);
-            else hm = weno5_dn(gm + 2);
+            else if (hy == 2) hm = weno7_dn<R, W>(gm + 1);
+            else hm = weno5_dn<R, W>(gm + 2);
         }
     }
 }
 
-template <int DIR, int SCH, bool FIRST>
-__global__ void __launch_bounds__(256) k_sweep(SweepArgs A) {
-    constexpr int NS = Tile<DIR>::NS, NA = Tile<DIR>::NA, NN = NS + 8, NF = NS + 1;
-    static_assert(NF * NA <= 256, "one face per thread");
-    __shared__ double sh[10 * NN * NA];
+// x sweep: a block owns 4 rows of 56 output nodes; the f+/f- of the 64 nodes
+// of a row (4 ghost nodes each side) are staged in LDS.
+constexpr int XS = 56, XA = 4, XN = XS + 8, XF = XS + 1;
+
+template <int SCH, bool FIRST, typename S>
+__global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
+    static_assert(XF * XA <= 256 && XN * XA == 256, "one node and one face per thread");
+    __shared__ S sh[10 * XN * XA];
     const Geo& g = A.g;
     const int tid = threadIdx.x;
-    // tile origin: s0 along the sweep, a0 across; k2 = the third index
-    int s0, a0, k2;
-    if (DIR == 0) { s0 = blockIdx.x * NS; a0 = blockIdx.y * NA; k2 = blockIdx.z; }
-    else { a0 = blockIdx.x * NA; s0 = blockIdx.y * NS; k2 = blockIdx.z; }
-    const int n_dir = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
-    const int n_acr = DIR == 0 ? g.ny : g.nx;
-    const long long sstride = DIR == 0 ? 1 : DIR == 1 ? (long long)g.ldx : g.sz;
-    // point (along s, across a) -> linear index
-    auto lin = [&](int s, int a) -> long long {
-        if (DIR == 0) return gidx(g, s, a, k2);
-        if (DIR == 1) return gidx(g, a, s, k2);
-        return gidx(g, a, k2, s);
-    };
+    const int s0 = blockIdx.x * XS, a0 = blockIdx.y * XA, k2 = blockIdx.z;
+    const int n_dir = g.nx;
+    // LDS element (v, n, a): node-contiguous so consecutive lanes hold
+    // consecutive x
+    auto at = [](int v, int n, int a) { return (v * XA + a) * XN + n; };
 
-    // ---- phase 1: split of NN x NA nodes into LDS
-    for (int idx = tid; idx < NN * NA; idx += 256) {
-        int n, a;
-        if (DIR == 0) { n = idx % NN; a = idx / NN; } else { a = idx % NA; n = idx / NA; }
+    // ---- phase 1: split of XN x XA nodes into LDS
+    {
+        const int n = tid % XN, a = tid / XN;
         const int s = s0 - 4 + n, ac = a0 + a;
-        double fp[5], fm[5];
-        if (s <= n_dir + 3 && ac < n_acr) {
-            const long long q = lin(s, ac);
-            const double d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
-            const double cc = sqrt(T) / A.c.ama;  // sound_speed_kernel, src/commen_kernel.cu:144-153
-            sw_split<double>(d * A.c.tmp0, u, v, w, cc, A.m1[q], A.m2[q], A.m3[q], A.c, fp, fm);
+        S fp[5], fm[5];
+        if (s <= n_dir + 3 && ac < g.ny) {
+            const long long q = gidx(g, s, ac, k2);
+            const S d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
+            const S cc = sqrt(T) * S(A.c.rama);  // sound_speed_kernel, src/commen_kernel.cu:144-153
+            sw_split<S>(d * S(A.c.tmp0), u, v, w, cc, S(A.m1[q]), S(A.m2[q]), S(A.m3[q]), A.c, fp, fm);
         } else {
 #pragma unroll
-            for (int m = 0; m < 5; m++) { fp[m] = 0.0; fm[m] = 0.0; }
+            for (int m = 0; m < 5; m++) { fp[m] = S(0); fm[m] = S(0); }
         }
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            sh[lds_at<DIR, NN, NA>(m, n, a)] = fp[m];
-            sh[lds_at<DIR, NN, NA>(5 + m, n, a)] = fm[m];
+            sh[at(m, n, a)] = fp[m];
+            sh[at(5 + m, n, a)] = fm[m];
         }
     }
     __syncthreads();
 
     // ---- phase 2: one half node per thread
-    double hp[5], hm[5];
+    S hp[5], hm[5];
     int f_loc = 0, a_loc = 0;
     bool fvalid = false;
-    if (tid < NF * NA) {
-        if (DIR == 0) { f_loc = tid % NF; a_loc = tid / NF; } else { a_loc = tid % NA; f_loc = tid / NA; }
+    if (tid < XF * XA) {
+        f_loc = tid % XF; a_loc = tid / XF;
         const int j = s0 - 1 + f_loc;  // face j+1/2
         const int ac = a0 + a_loc;
-        fvalid = (j <= n_dir - 1) && (ac < n_acr);
+        fvalid = (j <= n_dir - 1) && (ac < g.ny);
         if (fvalid) {
             int hy = 0;
-            if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[lin(j, ac)];
+            if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[gidx(g, j, ac, k2)];
             const int c = j + 1, r = j - n_dir;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                double gp[8], gm[8];
+                S gp[8], gm[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    gp[k] = sh[lds_at<DIR, NN, NA>(m, f_loc + k, a_loc)];
-                    gm[k] = sh[lds_at<DIR, NN, NA>(5 + m, f_loc + k, a_loc)];
+                    gp[k] = sh[at(m, f_loc + k, a_loc)];
+                    gm[k] = sh[at(5 + m, f_loc + k, a_loc)];
                 }
-                face_pm<double, SCH>(gp, gm, c, r, A.bl, A.br, hy, A.c.tv_lim, A.c.tv_max, hp[m], hm[m]);
+                face_pm<S, double, SCH>(gp, gm, c, r, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max), hp[m], hm[m]);
             }
         }
     }
     __syncthreads();
-    if (tid < NF * NA) {
+    if (tid < XF * XA) {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            sh[(m * NF + f_loc) * NA + a_loc] = fvalid ? hp[m] : 0.0;
-            sh[((5 + m) * NF + f_loc) * NA + a_loc] = fvalid ? hm[m] : 0.0;
+            sh[(m * XF + f_loc) * XA + a_loc] = fvalid ? hp[m] : S(0);
+            sh[((5 + m) * XF + f_loc) * XA + a_loc] = fvalid ? hm[m] : S(0);
         }
     }
     __syncthreads();
 
-    // ---- phase 3: du at nodes, in the reference's order P then M
-    if (tid < NS * NA) {
-        int n, a;
-        if (DIR == 0) { n = tid % NS; a = tid / NS; } else { a = tid % NA; n = tid / NA; }
+    // ---- phase 3: du at nodes, in the reference's order P then M; the
+    // Jacobian factor and the accumulation are in double
+    if (tid < XS * XA) {
+        const int n = tid % XS, a = tid / XS;
         const int i = s0 + n, ac = a0 + a;
-        if (i < n_dir && ac < n_acr) {
-            const long long q = lin(i, ac);
+        if (i < n_dir && ac < g.ny) {
+            const long long q = gidx(g, i, ac, k2);
             const double ajac = A.jac[q];
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
             const double rh = 1.0 / A.h;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const double pr = sh[(m * NF + n + 1) * NA + a], pl = sh[(m * NF + n) * NA + a];
-                const double mr = sh[((5 + m) * NF + n + 1) * NA + a], ml = sh[((5 + m) * NF + n) * NA + a];
-                const double tP = zp ? 0.0 : -ajac * (pr - pl) * rh;
-                const double tM = zm ? 0.0 : -ajac * (mr - ml) * rh;
-                double* dv = A.du + (long long)m * A.fs;
-                double acc = FIRST ? 0.0 : dv[q];
+                const S pr = sh[(m * XF + n + 1) * XA + a], pl = sh[(m * XF + n) * XA + a];
+                const S mr = sh[((5 + m) * XF + n + 1) * XA + a], ml = sh[((5 + m) * XF + n) * XA + a];
+                const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
+                const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
+                S* dv = A.du + (long long)m * A.fs;
+                double acc = FIRST ? 0.0 : double(dv[q]);
                 acc = acc + tP;
                 acc = acc + tM;
-                dv[q] = acc;
+                dv[q] = S(acc);
             }
         }
     }
-    (void)sstride;
+}
+
+// y and z sweeps with LDS tiles: a block owns NA x-consecutive columns (one
+// NA*8-byte row segment per node) and NS output nodes along the sweep, for one
+// value of the third index.  Same three phases as the x sweep; a thread may
+// handle up to two faces and two nodes.
+template <int DIR, int SCH, int NS, int NA, typename S>
+__global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
+    static_assert(DIR == 1 || DIR == 2, "x is swept by k_sweep");
+    constexpr int NN = NS + 8, NF = NS + 1;
+    constexpr int FPT = (NF * NA + 255) / 256;
+    static_assert(FPT <= 2 && NN * NA >= NF * NA, "at most two faces per thread");
+    __shared__ S sh[10 * NN * NA];
+    const Geo& g = A.g;
+    const int tid = threadIdx.x;
+    const int a0 = blockIdx.x * NA, s0 = blockIdx.y * NS, k2 = blockIdx.z;
+    const int n_dir = DIR == 1 ? g.ny : g.nz;
+    auto lin = [&](int s, int a) -> long long { return DIR == 1 ? gidx(g, a, s, k2) : gidx(g, a, k2, s); };
+    auto at = [](int v, int n, int a) { return (v * NN + n) * NA + a; };
+
+    // ---- phase 1: split of NN x NA nodes into LDS
+    for (int idx = tid; idx < NN * NA; idx += 256) {
+        const int a = idx % NA, n = idx / NA;
+        const int s = s0 - 4 + n, ac = a0 + a;
+        S fp[5], fm[5];
+        if (s <= n_dir + 3 && ac < g.nx) {
+            const long long q = lin(s, ac);
+            const S d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
+            const S cc = sqrt(T) * S(A.c.rama);  // sound_speed_kernel, src/commen_kernel.cu:144-153
+            sw_split<S>(d * S(A.c.tmp0), u, v, w, cc, S(A.m1[q]), S(A.m2[q]), S(A.m3[q]), A.c, fp, fm);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) { fp[m] = S(0); fm[m] = S(0); }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            sh[at(m, n, a)] = fp[m];
+            sh[at(5 + m, n, a)] = fm[m];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: half nodes
+    S hp[FPT][5], hm[FPT][5];
+#pragma unroll
+    for (int t = 0; t < FPT; t++) {
+        const int idx = tid + 256 * t;
+        if (idx >= NF * NA) break;
+        const int a_loc = idx % NA, f_loc = idx / NA;
+        const int j = s0 - 1 + f_loc;  // face j+1/2
+        const int ac = a0 + a_loc;
+        if (j <= n_dir - 1 && ac < g.nx) {
+            int hy = 0;
+            if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[lin(j, ac)];
+            const int c = j + 1, r = j - n_dir;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                S gp[8], gm[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    gp[k] = sh[at(m, f_loc + k, a_loc)];
+                    gm[k] = sh[at(5 + m, f_loc + k, a_loc)];
+                }
+                face_pm<S, double, SCH>(gp, gm, c, r, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max), hp[t][m],
+                                        hm[t][m]);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) { hp[t][m] = S(0); hm[t][m] = S(0); }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < FPT; t++) {
+        const int idx = tid + 256 * t;
+        if (idx >= NF * NA) break;
+        const int a_loc = idx % NA, f_loc = idx / NA;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            sh[(m * NF + f_loc) * NA + a_loc] = hp[t][m];
+            sh[((5 + m) * NF + f_loc) * NA + a_loc] = hm[t][m];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: du at nodes (P then M; Jacobian factor and sum in double)
+    const double rh = 1.0 / A.h;
+    for (int idx = tid; idx < NS * NA; idx += 256) {
+        const int a = idx % NA, n = idx / NA;
+        const int i = s0 + n, ac = a0 + a;
+        if (i < n_dir && ac < g.nx) {
+            const long long q = lin(i, ac);
+            const double ajac = A.jac[q];
+            const bool zp = A.nrp && i == 0;
+            const bool zm = A.nrm && i == n_dir - 1;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const S pr = sh[(m * NF + n + 1) * NA + a], pl = sh[(m * NF + n) * NA + a];
+                const S mr = sh[((5 + m) * NF + n + 1) * NA + a], ml = sh[((5 + m) * NF + n) * NA + a];
+                const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
+                const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
+                S* dv = A.du + (long long)m * A.fs;
+                double acc = double(dv[q]);
+                acc = acc + tP;
+                acc = acc + tM;
+                dv[q] = S(acc);
+            }
+        }
+    }
 }
 
 // y and z sweeps: one thread per (x, across) column marching along the sweep
 // direction over a chunk of L output nodes.  The f+/f- of the 8 nodes a face
 // needs (j-3..j+4) live in a register queue: each node is split once per chunk
 // (plus the 8 warm-up nodes), rows of 64 x-consecutive nodes are read by one
-// wave (512 contiguous bytes), and there is no LDS and no barrier.  The next
-// node's inputs are fetched one iteration ahead.
-template <int DIR, int SCH, int L>
-__global__ void __launch_bounds__(256) k_march(SweepArgs A) {
+// wave (512 contiguous bytes in FP64), and there is no LDS and no barrier.
+// The next node's inputs are fetched one iteration ahead.
+template <int DIR, int SCH, int L, typename S>
+__global__ void __launch_bounds__(256) k_march(SweepArgsT<S> A) {
     static_assert(DIR == 1 || DIR == 2, "x is swept by k_sweep");
     const Geo& g = A.g;
     const int i = blockIdx.x * 64 + threadIdx.x;
@@ -258,27 +364,27 @@ __global__ void __launch_bounds__(256) k_march(SweepArgs A) {
     const int ic = min(i, g.nx - 1), acl = min(a, n_acr - 1);
     const long long st = DIR == 1 ? (long long)g.ldx : g.sz;
     const long long base = DIR == 1 ? gidx(g, ic, 0, acl) : gidx(g, ic, acl, 0);
-    const double* __restrict__ pd = A.d;
-    const double* __restrict__ pu = A.u;
-    const double* __restrict__ pv = A.v;
-    const double* __restrict__ pw = A.w;
-    const double* __restrict__ pT = A.T;
+    const S* __restrict__ pd = A.d;
+    const S* __restrict__ pu = A.u;
+    const S* __restrict__ pv = A.v;
+    const S* __restrict__ pw = A.w;
+    const S* __restrict__ pT = A.T;
     const double* __restrict__ p1 = A.m1;
     const double* __restrict__ p2 = A.m2;
     const double* __restrict__ p3 = A.m3;
     const double rh = 1.0 / A.h;
 
-    double qp[5][8], qm[5][8];
-    double in[8];
+    S qp[5][8], qm[5][8];
+    S in[8];
     auto fetch = [&](int s) {
         const long long q = base + (long long)s * st;
         in[0] = pd[q]; in[1] = pu[q]; in[2] = pv[q]; in[3] = pw[q]; in[4] = pT[q];
-        in[5] = p1[q]; in[6] = p2[q]; in[7] = p3[q];
+        in[5] = S(p1[q]); in[6] = S(p2[q]); in[7] = S(p3[q]);
     };
     auto split_into = [&](int slot) {
-        const double cc = sqrt(in[4]) / A.c.ama;  // sound_speed_kernel, src/commen_kernel.cu:144-153
-        double fp[5], fm[5];
-        sw_split<double>(in[0] * A.c.tmp0, in[1], in[2], in[3], cc, in[5], in[6], in[7], A.c, fp, fm);
+        const S cc = sqrt(in[4]) * S(A.c.rama);  // sound_speed_kernel, src/commen_kernel.cu:144-153
+        S fp[5], fm[5];
+        sw_split<S>(in[0] * S(A.c.tmp0), in[1], in[2], in[3], cc, in[5], in[6], in[7], A.c, fp, fm);
 #pragma unroll
         for (int m = 0; m < 5; m++) { qp[m][slot] = fp[m]; qm[m][slot] = fm[m]; }
     };
@@ -288,17 +394,17 @@ __global__ void __launch_bounds__(256) k_march(SweepArgs A) {
         split_into(t);
     }
     fetch(s0 + 3);
-    double pp[5] = {0, 0, 0, 0, 0}, pm[5] = {0, 0, 0, 0, 0};
+    S pp[5] = {0, 0, 0, 0, 0}, pm[5] = {0, 0, 0, 0, 0};
     for (int j = s0 - 1; j < s1; j++) {
         split_into(7);
         if (j + 5 <= n_dir + 3) fetch(j + 5);
         int hy = 0;
         if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[base + (long long)j * st];
-        double hp[5], hm[5];
+        S hp[5], hm[5];
 #pragma unroll
         for (int m = 0; m < 5; m++)
-            face_pm<double, SCH>(qp[m], qm[m], j + 1, j - n_dir, A.bl, A.br, hy, A.c.tv_lim, A.c.tv_max, hp[m],
-                                 hm[m]);
+            face_pm<S, double, SCH>(qp[m], qm[m], j + 1, j - n_dir, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max),
+                                    hp[m], hm[m]);
         if (j >= s0 && own) {
             const long long q = base + (long long)j * st;
             const double ajac = A.jac[q];
@@ -306,13 +412,13 @@ __global__ void __launch_bounds__(256) k_march(SweepArgs A) {
             const bool zm = A.nrm && j == n_dir - 1;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const double tP = zp ? 0.0 : -ajac * (hp[m] - pp[m]) * rh;
-                const double tM = zm ? 0.0 : -ajac * (hm[m] - pm[m]) * rh;
-                double* dv = A.du + (long long)m * A.fs;
-                double acc = dv[q];
+                const double tP = zp ? 0.0 : -ajac * double(hp[m] - pp[m]) * rh;
+                const double tM = zm ? 0.0 : -ajac * double(hm[m] - pm[m]) * rh;
+                S* dv = A.du + (long long)m * A.fs;
+                double acc = double(dv[q]);
                 acc = acc + tP;
                 acc = acc + tM;
-                dv[q] = acc;
+                dv[q] = S(acc);
             }
         }
 #pragma unroll
@@ -326,27 +432,51 @@ __global__ void __launch_bounds__(256) k_march(SweepArgs A) {
 }
 
 #ifndef OCFD_LMARCH
-#define OCFD_LMARCH 32  // output nodes per marching chunk (y and z sweeps)
+#define OCFD_LMARCH 32  // output nodes per marching chunk (k_march)
+#endif
+// y/z tile shapes (output nodes x columns): rows of 64 bytes suit both
+// precisions (8 doubles, 16 floats)
+#ifndef OCFD_YZ_NS_D
+#define OCFD_YZ_NS_D 28
+#endif
+#ifndef OCFD_YZ_NA_D
+#define OCFD_YZ_NA_D 8
+#endif
+#ifndef OCFD_YZ_NS_F
+#define OCFD_YZ_NS_F 15
+#endif
+#ifndef OCFD_YZ_NA_F
+#define OCFD_YZ_NA_F 16
+#endif
+#ifndef OCFD_MARCH_F32
+#define OCFD_MARCH_F32 0  // y/z sweeps of the float path: 1 = k_march, 0 = k_tile
 #endif
 constexpr int LMARCH = OCFD_LMARCH;
+template <typename S> struct YZ { static constexpr int NS = OCFD_YZ_NS_D, NA = OCFD_YZ_NA_D; };
+template <> struct YZ<float> { static constexpr int NS = OCFD_YZ_NS_F, NA = OCFD_YZ_NA_F; };
 
-template <int DIR, int SCH>
-void launch_dir(const SweepArgs& a, bool first, hipStream_t st) {
+template <int DIR, int SCH, typename S>
+void launch_dir(const SweepArgsT<S>& a, bool first, hipStream_t st) {
     const Geo& g = a.g;
     if constexpr (DIR == 0) {
-        constexpr int NS = Tile<0>::NS, NA = Tile<0>::NA;
-        dim3 grid((g.nx + NS - 1) / NS, (g.ny + NA - 1) / NA, g.nz);
-        if (first) hipLaunchKernelGGL((k_sweep<0, SCH, true>), grid, dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_sweep<0, SCH, false>), grid, dim3(256), 0, st, a);
+        dim3 grid((g.nx + XS - 1) / XS, (g.ny + XA - 1) / XA, g.nz);
+        if (first) hipLaunchKernelGGL((k_sweep<SCH, true, S>), grid, dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_sweep<SCH, false, S>), grid, dim3(256), 0, st, a);
     } else {
         const int n_dir = DIR == 1 ? g.ny : g.nz, n_acr = DIR == 1 ? g.nz : g.ny;
-        dim3 grid((g.nx + 63) / 64, (n_acr + 3) / 4, (n_dir + LMARCH - 1) / LMARCH);
-        hipLaunchKernelGGL((k_march<DIR, SCH, LMARCH>), grid, dim3(64, 4), 0, st, a);
+        if constexpr (std::is_same<S, float>::value && OCFD_MARCH_F32) {
+            dim3 grid((g.nx + 63) / 64, (n_acr + 3) / 4, (n_dir + LMARCH - 1) / LMARCH);
+            hipLaunchKernelGGL((k_march<DIR, SCH, LMARCH, S>), grid, dim3(64, 4), 0, st, a);
+        } else {
+            constexpr int NS = YZ<S>::NS, NA = YZ<S>::NA;
+            dim3 grid((g.nx + NA - 1) / NA, (n_dir + NS - 1) / NS, n_acr);
+            hipLaunchKernelGGL((k_tile<DIR, SCH, NS, NA, S>), grid, dim3(256), 0, st, a);
+        }
     }
 }
 
-template <int SCH>
-void launch_scheme(int dir, const SweepArgs& a, bool first, hipStream_t st) {
+template <int SCH, typename S>
+void launch_scheme(int dir, const SweepArgsT<S>& a, bool first, hipStream_t st) {
     if (dir == 0) launch_dir<0, SCH>(a, first, st);
     else if (dir == 1) launch_dir<1, SCH>(a, first, st);
     else launch_dir<2, SCH>(a, first, st);
@@ -354,7 +484,8 @@ void launch_scheme(int dir, const SweepArgs& a, bool first, hipStream_t st) {
 
 }  // namespace
 
-int launch_sweep(int scheme, int dir, const SweepArgs& a, bool first, hipStream_t st) {
+template <typename S>
+int launch_sweep(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStream_t st) {
     switch (scheme) {
     case OCFD_UP7: launch_scheme<OCFD_UP7>(dir, a, first, st); break;
     case OCFD_WENO5: launch_scheme<OCFD_WENO5>(dir, a, first, st); break;
@@ -370,3 +501,5 @@ int launch_sweep(int scheme, int dir, const SweepArgs& a, bool first, hipStream_
     }
     return 0;
 }
+template int launch_sweep<double>(int, int, const SweepArgsT<double>&, bool, hipStream_t);
+template int launch_sweep<float>(int, int, const SweepArgsT<float>&, bool, hipStream_t);

@@ -1,6 +1,10 @@
 // k_point.hip - pointwise and boundary kernels (gfx950): RK stage fused with
 // cons->prim and viscosity, periodic face fills, IBC 108, the face refresh,
 // the hybrid_auto sensor, the shock-capturing filter and the metric kernels.
+//
+// S (double or float) is the storage type of the primitives and the
+// viscosity; the conserved state, the metrics and all arithmetic here are
+// double.
 #include "ocfd_dev.h"
 #include "ocfd_kernels.h"
 
@@ -11,7 +15,8 @@ namespace {
 // (src/OCFD_mpi_dev.cu:186-310): ghost[n..n+3] <- interior[0..3],
 // ghost[-4..-1] <- interior[n-4..n-1], other two indices interior only.
 // ---------------------------------------------------------------------------
-__global__ void k_fill(Geo g, double* base, int nf, long long fs, int dir) {
+template <typename T>
+__global__ void k_fill(Geo g, T* base, int nf, long long fs, int dir) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;  // position in the 8 x a x b slab
     const int na = dir == 0 ? g.ny : g.nx;
     const int nb = dir == 2 ? g.ny : g.nz;
@@ -35,7 +40,8 @@ __global__ void k_fill(Geo g, double* base, int nf, long long fs, int dir) {
 // with cons->prim + p (cons_to_pri_kernel, src/commen_kernel.cu:77-109) and
 // Sutherland's law (get_Amu_kernal, src/commen_kernel.cu:122-131).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_stage(StageArgs A) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_stage(StageArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
     const long long q = gidx(A.g, i, j, k), fs = A.fs;
@@ -44,15 +50,16 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs A) {
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         const long long o = m * fs + q;
+        const double du = double(A.du[o]);
         double r;
         if (A.krk == 1) {
-            r = A.fn[o] + dt * A.du[o];
+            r = A.fn[o] + dt * du;
         } else if (A.krk == 2) {
             const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * A.du[o]);
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du);
         } else {
             const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * A.du[o]);
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du);
         }
         A.f[o] = r;
         if (A.flap) A.flap[o] = r;
@@ -61,9 +68,9 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs A) {
     const double d1 = fv[0];
     const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
     const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
-    A.d[q] = d1; A.u[q] = u1; A.v[q] = v1; A.w[q] = w1; A.T[q] = T2;
-    A.P[q] = T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama);
-    A.amu[q] = A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2);
+    A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
+    A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
+    A.amu[q] = S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2));
 }
 
 // ---------------------------------------------------------------------------
@@ -71,40 +78,47 @@ __global__ void __launch_bounds__(256) k_stage(StageArgs A) {
 // 27-179), launched in the reference's order: inlet, top, wall velocity, wall
 // temperature/density with mirrored ghosts.  Interior indices; i<0 is ghost.
 // ---------------------------------------------------------------------------
-__global__ void k_bc_inlet(BCArgs A) {  // do_ub1_inlet_kernel
+template <typename S>
+__global__ void k_bc_inlet(BCArgsT<S> A) {  // do_ub1_inlet_kernel
     const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (j >= A.g.ny) return;
     const int ny = A.g.ny;
     for (int i = -4; i <= 0; i++) {
         const long long q = gidx(A.g, i, j, k);
-        A.d[q] = A.ub1[j]; A.u[q] = A.ub1[j + ny]; A.v[q] = A.ub1[j + 2 * ny]; A.w[q] = 0.; A.T[q] = A.ub1[j + 3 * ny];
+        A.d[q] = S(A.ub1[j]); A.u[q] = S(A.ub1[j + ny]); A.v[q] = S(A.ub1[j + 2 * ny]); A.w[q] = S(0);
+        A.T[q] = S(A.ub1[j + 3 * ny]);
     }
 }
-__global__ void k_bc_top(BCArgs A) {  // do_ub1_top_kernel
+template <typename S>
+__global__ void k_bc_top(BCArgsT<S> A) {  // do_ub1_top_kernel
     const int i = A.x_begin + blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (i >= A.g.nx) return;
     const int ny = A.g.ny;
     const long long q = gidx(A.g, i, ny - 1, k);
-    A.d[q] = A.ub1[ny - 1]; A.u[q] = A.ub1[ny + ny - 1]; A.v[q] = A.ub1[2 * ny + ny - 1]; A.w[q] = 0.;
-    A.T[q] = A.ub1[3 * ny + ny - 1];
+    A.d[q] = S(A.ub1[ny - 1]); A.u[q] = S(A.ub1[ny + ny - 1]); A.v[q] = S(A.ub1[2 * ny + ny - 1]); A.w[q] = S(0);
+    A.T[q] = S(A.ub1[3 * ny + ny - 1]);
 }
-__global__ void k_bc_wall_vel(BCArgs A) {  // do_wall_dist_kernel
+template <typename S>
+__global__ void k_bc_wall_vel(BCArgsT<S> A) {  // do_wall_dist_kernel
     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (i >= A.g.nx) return;
     const long long q = gidx(A.g, i, 0, k);
-    A.u[q] = 0.;
-    A.v[q] = A.epsl * A.fx[i] * A.gz[k] * A.ht;
-    A.w[q] = 0.;
+    A.u[q] = S(0);
+    A.v[q] = S(A.epsl * A.fx[i] * A.gz[k] * A.ht);
+    A.w[q] = S(0);
 }
-__global__ void k_bc_wall_T(BCArgs A) {  // do_wall_Tp1_kernel / do_wall_Tp2_kernel
+template <typename S>
+__global__ void k_bc_wall_T(BCArgsT<S> A) {  // do_wall_Tp1_kernel / do_wall_Tp2_kernel
     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (i >= A.g.nx) return;
     const Geo& g = A.g;
     const long long q0 = gidx(g, i, 0, k), q1 = gidx(g, i, 1, k), q2 = gidx(g, i, 2, k), q3 = gidx(g, i, 3, k);
+    auto D = [&](long long q) { return double(A.d[q]); };
+    auto T = [&](long long q) { return double(A.T[q]); };
     if (A.tw > 0) {
-        const double pw = (18. * A.d[q1] * A.T[q1] - 9. * A.d[q2] * A.T[q2] + 2. * A.d[q3] * A.T[q3]) / 11.;
-        A.T[q0] = A.tw;
-        A.d[q0] = pw / A.T[q0];
+        const double pw = (18. * D(q1) * T(q1) - 9. * D(q2) * T(q2) + 2. * D(q3) * T(q3)) / 11.;
+        A.T[q0] = S(A.tw);
+        A.d[q0] = S(pw / T(q0));
         for (int l = 0; l < 4; l++) {  // ghost j = l-4 mirrors interior j = 4-l
             const long long qg = gidx(g, i, l - 4, k), qm = gidx(g, i, 4 - l, k);
             A.d[qg] = A.d[qm];
@@ -114,14 +128,15 @@ __global__ void k_bc_wall_T(BCArgs A) {  // do_wall_Tp1_kernel / do_wall_Tp2_ker
             A.T[qg] = A.T[qm];
         }
     } else {
-        A.T[q0] = (18. * A.T[q1] - 9. * A.T[q2] + 2. * A.T[q3]) / 11.;
-        const double pw = (18. * A.d[q1] * A.T[q1] - 9. * A.d[q2] * A.T[q2] + 2. * A.d[q3] * A.T[q3]) / 11.;
-        A.d[q0] = pw / A.T[q0];
+        A.T[q0] = S((18. * T(q1) - 9. * T(q2) + 2. * T(q3)) / 11.);
+        const double pw = (18. * D(q1) * T(q1) - 9. * D(q2) * T(q2) + 2. * D(q3) * T(q3)) / 11.;
+        A.d[q0] = S(pw / T(q0));
     }
 }
 // Viscosity on the faces IBC 108 may change (get_Amu runs after OCFD_bc,
 // src/OCFD_NS_Solver.cu:83-87): x=0, y=0, y=ny-1.
-__global__ void k_bc_amu(BCArgs A) {
+template <typename S>
+__global__ void k_bc_amu(BCArgsT<S> A) {
     const Geo& g = A.g;
     const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     int i, j;
@@ -130,13 +145,14 @@ __global__ void k_bc_amu(BCArgs A) {
     else if (t < g.ny + 2 * g.nx) { i = t - g.ny - g.nx; j = g.ny - 1; }
     else return;
     const long long q = gidx(g, i, j, k);
-    const double T = A.T[q];
-    A.amu[q] = A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T);
+    const double T = double(A.T[q]);
+    A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
 }
 
 // Face refresh of OCFD_bc (src/OCFD_boundary.cu:40-91): prim->cons
 // (pri_to_cons_kernel, src/commen_kernel.cu:32-64) on all six faces.
-__device__ __forceinline__ void p2c(const BCArgs& A, long long q) {
+template <typename S>
+__device__ __forceinline__ void p2c(const BCArgsT<S>& A, long long q) {
     const double d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
     A.f[q] = d;
     A.f[A.fs + q] = u * d;
@@ -144,7 +160,8 @@ __device__ __forceinline__ void p2c(const BCArgs& A, long long q) {
     A.f[3 * A.fs + q] = w * d;
     A.f[4 * A.fs + q] = d * ((u * u + v * v + w * w) * 0.5 + A.c.cv * T);
 }
-__global__ void k_face_refresh(BCArgs A, int dir) {
+template <typename S>
+__global__ void k_face_refresh(BCArgsT<S> A, int dir) {
     const Geo& g = A.g;
     const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,12 +183,13 @@ __global__ void k_face_refresh(BCArgs A, int dir) {
 // (dp>i2).  Node (i,j,k) with i,j,k >= -1 serves the x flags when j,k >= 0,
 // the y flags when i,k >= 0 and the z flags when i,j >= 0.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const double* __restrict__ d,
-                                                const double* __restrict__ T, int8_t* fx, int8_t* fy, int8_t* fz) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const S* __restrict__ d,
+                                                const S* __restrict__ T, int8_t* fx, int8_t* fy, int8_t* fz) {
     const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1, k = blockIdx.z - 1;
     if (i >= g.nx || j >= g.ny) return;
     const long long q = gidx(g, i, j, k);
-    auto P = [&](long long o) { return c.p00 * d[o] * T[o]; };
+    auto P = [&](long long o) { return c.p00 * double(d[o]) * double(T[o]); };
     const double pc = P(q);
     const long long sx = 1, sy = g.ldx, sz = g.sz;
     double pm = P(q - sx), pp = P(q + sx);
@@ -192,7 +210,8 @@ __global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const double* _
 // Shock-capturing filter (filter_{x,y}3d_shock_kernel, src/OCFD_filtering.cu:
 // 366-416, 451-495) for one direction over a box of output points.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double filt_sigma(const double* __restrict__ P, long long L, long long st, double rth) {
+template <typename S>
+__device__ __forceinline__ double filt_sigma(const S* __restrict__ P, long long L, long long st, double rth) {
     const double p_m2 = P[L - 2 * st], p_m1 = P[L - st], p_00 = P[L], p_p1 = P[L + st], p_p2 = P[L + 2 * st];
     const double dp0 = 0.25 * (-p_p1 + 2.0 * p_00 - p_m1);
     const double dp1 = 0.25 * (-p_p2 + 2.0 * p_p1 - p_00);
@@ -201,7 +220,8 @@ __device__ __forceinline__ double filt_sigma(const double* __restrict__ P, long 
     ri = 1.0 - rth / ri;
     return 0.5 * (ri + fabs(ri));
 }
-__global__ void __launch_bounds__(256) k_filter(FilterArgs A) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_filter(FilterArgsT<S> A) {
     const int i = A.lo[0] + blockIdx.x * 64 + threadIdx.x, j = A.lo[1] + blockIdx.y * 4 + threadIdx.y;
     const int k = A.lo[2] + blockIdx.z;
     if (i >= A.hi[0] || j >= A.hi[1] || k >= A.hi[2]) return;
@@ -297,28 +317,31 @@ __global__ void __launch_bounds__(256) k_cofactors(Geo g, const double* __restri
 
 // Whole-interior prim->cons (src/OCFD_init.cu:99-110) and viscosity
 // (get_Amu, src/commen_kernel.cu:133-140), used at initialisation.
-__global__ void __launch_bounds__(256) k_p2c_all(BCArgs A) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_p2c_all(BCArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
     p2c(A, gidx(A.g, i, j, k));
 }
 // cons->prim + p at every interior point (get_duvwT, src/commen_kernel.cu:111-118).
-__global__ void __launch_bounds__(256) k_c2p_all(StageArgs A) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_c2p_all(StageArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
     const long long q = gidx(A.g, i, j, k), fs = A.fs;
     const double d1 = A.f[q];
     const double u1 = A.f[fs + q] / d1, v1 = A.f[2 * fs + q] / d1, w1 = A.f[3 * fs + q] / d1;
     const double T2 = (A.f[4 * fs + q] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
-    A.d[q] = d1; A.u[q] = u1; A.v[q] = v1; A.w[q] = w1; A.T[q] = T2;
-    A.P[q] = T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama);
+    A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
+    A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
 }
-__global__ void __launch_bounds__(256) k_amu_all(BCArgs A) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_amu_all(BCArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
     const long long q = gidx(A.g, i, j, k);
-    const double T = A.T[q];
-    A.amu[q] = A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T);
+    const double T = double(A.T[q]);
+    A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
 }
 
 __global__ void k_set(double* p, long long n, double v) {
@@ -326,52 +349,64 @@ __global__ void k_set(double* p, long long n, double v) {
     if (t < n) p[t] = v;
 }
 
+template <typename To, typename From>
+__global__ void k_convert(To* dst, const From* src, long long n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = To(src[t]);
+}
+
 dim3 pgrid(const Geo& g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
 
 }  // namespace
 
-void launch_fill(const Geo& g, double* base, int nf, long long fs, int dir, hipStream_t st) {
+template <typename T>
+void launch_fill(const Geo& g, T* base, int nf, long long fs, int dir, hipStream_t st) {
     const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
     const long long tot = 8LL * na * nb;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, g, base, nf, fs, dir);
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, g, base, nf, fs, dir);
 }
 
-void launch_stage(const StageArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_stage, pgrid(a.g), dim3(64, 4), 0, st, a);
+template <typename S>
+void launch_stage(const StageArgsT<S>& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_stage<S>, pgrid(a.g), dim3(64, 4), 0, st, a);
 }
 
-void launch_bc108(const BCArgs& a, hipStream_t st) {
+template <typename S>
+void launch_bc108(const BCArgsT<S>& a, hipStream_t st) {
     const Geo& g = a.g;
-    hipLaunchKernelGGL(k_bc_inlet, dim3((g.ny + 63) / 64, g.nz), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc_inlet<S>, dim3((g.ny + 63) / 64, g.nz), dim3(64), 0, st, a);
     if (g.nx - a.x_begin > 0)
-        hipLaunchKernelGGL(k_bc_top, dim3((g.nx - a.x_begin + 63) / 64, g.nz), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(k_bc_wall_vel, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
-    if (a.wall_not_normal == 0) hipLaunchKernelGGL(k_bc_wall_T, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_bc_top<S>, dim3((g.nx - a.x_begin + 63) / 64, g.nz), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc_wall_vel<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
+    if (a.wall_not_normal == 0) hipLaunchKernelGGL(k_bc_wall_T<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
 }
 
-void launch_face_refresh(const BCArgs& a, hipStream_t st) {
+template <typename S>
+void launch_face_refresh(const BCArgsT<S>& a, hipStream_t st) {
     const Geo& g = a.g;
     if (a.amu) {
         const int nt = g.ny + 2 * g.nx;
-        hipLaunchKernelGGL(k_bc_amu, dim3((nt + 63) / 64, g.nz), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(k_bc_amu<S>, dim3((nt + 63) / 64, g.nz), dim3(64), 0, st, a);
     }
     for (int dir = 0; dir < 3; dir++) {
         const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
         const long long tot = 2LL * na * nb;
-        hipLaunchKernelGGL(k_face_refresh, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, dir);
+        hipLaunchKernelGGL(k_face_refresh<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, dir);
     }
 }
 
-void launch_sensor(const Geo& g, const Consts& c, const double* d, const double* T, int8_t* fx, int8_t* fy,
-                   int8_t* fz, hipStream_t st) {
+template <typename S>
+void launch_sensor(const Geo& g, const Consts& c, const S* d, const S* T, int8_t* fx, int8_t* fy, int8_t* fz,
+                   hipStream_t st) {
     dim3 grid((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
-    hipLaunchKernelGGL(k_sensor, grid, dim3(64, 4), 0, st, g, c, d, T, fx, fy, fz);
+    hipLaunchKernelGGL(k_sensor<S>, grid, dim3(64, 4), 0, st, g, c, d, T, fx, fy, fz);
 }
 
-void launch_filter(const FilterArgs& a, hipStream_t st) {
+template <typename S>
+void launch_filter(const FilterArgsT<S>& a, hipStream_t st) {
     const int ex = a.hi[0] - a.lo[0], ey = a.hi[1] - a.lo[1], ez = a.hi[2] - a.lo[2];
     if (ex <= 0 || ey <= 0 || ez <= 0) return;
-    hipLaunchKernelGGL(k_filter, dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(64, 4), 0, st, a);
+    hipLaunchKernelGGL(k_filter<S>, dim3((ex + 63) / 64, (ey + 3) / 4, ez), dim3(64, 4), 0, st, a);
 }
 
 void launch_metric_deriv(const Geo& g, const double* coord, int dir, int bound, double* out, hipStream_t st) {
@@ -384,19 +419,44 @@ void launch_cofactors(const Geo& g, const double* D9, long long fs, double* cons
     hipLaunchKernelGGL(k_cofactors, pgrid(g), dim3(64, 4), 0, st, g, D9, fs, o);
 }
 
-void launch_p2c_all(const BCArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_p2c_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+template <typename S>
+void launch_p2c_all(const BCArgsT<S>& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_p2c_all<S>, pgrid(a.g), dim3(64, 4), 0, st, a);
 }
 
-void launch_c2p_all(const StageArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_c2p_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+template <typename S>
+void launch_c2p_all(const StageArgsT<S>& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_c2p_all<S>, pgrid(a.g), dim3(64, 4), 0, st, a);
 }
 
-void launch_amu_all(const BCArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_amu_all, pgrid(a.g), dim3(64, 4), 0, st, a);
+template <typename S>
+void launch_amu_all(const BCArgsT<S>& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_amu_all<S>, pgrid(a.g), dim3(64, 4), 0, st, a);
 }
 
 void launch_set(const Geo& g, double* base, long long n, double value, hipStream_t st) {
     (void)g;
     hipLaunchKernelGGL(k_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, base, n, value);
 }
+
+template <typename To, typename From>
+void launch_convert(To* dst, const From* src, long long n, hipStream_t st) {
+    hipLaunchKernelGGL((k_convert<To, From>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+}
+
+#define OCFD_INSTANTIATE(S)                                                                                     \
+    template void launch_fill<S>(const Geo&, S*, int, long long, int, hipStream_t);                          \
+    template void launch_stage<S>(const StageArgsT<S>&, hipStream_t);                                        \
+    template void launch_bc108<S>(const BCArgsT<S>&, hipStream_t);                                           \
+    template void launch_face_refresh<S>(const BCArgsT<S>&, hipStream_t);                                    \
+    template void launch_sensor<S>(const Geo&, const Consts&, const S*, const S*, int8_t*, int8_t*, int8_t*, \
+                                   hipStream_t);                                                              \
+    template void launch_filter<S>(const FilterArgsT<S>&, hipStream_t);                                      \
+    template void launch_p2c_all<S>(const BCArgsT<S>&, hipStream_t);                                         \
+    template void launch_c2p_all<S>(const StageArgsT<S>&, hipStream_t);                                      \
+    template void launch_amu_all<S>(const BCArgsT<S>&, hipStream_t);
+OCFD_INSTANTIATE(double)
+OCFD_INSTANTIATE(float)
+#undef OCFD_INSTANTIATE
+template void launch_convert<float, double>(float*, const double*, long long, hipStream_t);
+template void launch_convert<double, float>(double*, const float*, long long, hipStream_t);

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ocfd_kernels.h"
@@ -50,10 +51,17 @@ struct ocfd_ctx {
     long long fs = 0;  // elements per field
     // device memory
     std::vector<void*> allocs;
+    // primitives, viscosity, right-hand side and viscous fluxes: double on
+    // the plain path; float (the s* pointers) on the mixed path
+    bool mixed = false;
     double *d = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *T = nullptr, *P = nullptr;
+    double *du = nullptr, *amu = nullptr, *ev = nullptr;
+    float *sd = nullptr, *su = nullptr, *sv = nullptr, *sw = nullptr, *sT = nullptr, *sP = nullptr;
+    float *sdu = nullptr, *samu = nullptr, *sev = nullptr;
+    // always double: metrics, conserved state, filter input, scratch
     double* M[M_COUNT] = {};
-    double *f = nullptr, *fn = nullptr, *du = nullptr, *flap = nullptr, *amu = nullptr;
-    double *grad = nullptr, *ev = nullptr;
+    double *f = nullptr, *fn = nullptr, *flap = nullptr;
+    double* grad = nullptr;  // 12 fields of scratch (metric set-up, precision conversions)
     int8_t* flag[3] = {};
     double *ub1 = nullptr, *fxd = nullptr, *gzd = nullptr;
     // host state
@@ -77,6 +85,19 @@ struct ocfd_ctx {
 };
 
 namespace {
+
+// The arrays whose type follows the precision of the context.
+template <typename S>
+struct View {
+    S *d, *u, *v, *w, *T, *P, *amu, *du, *ev;
+};
+template <typename S> View<S> view(ocfd_ctx* x);
+template <> View<double> view<double>(ocfd_ctx* x) {
+    return {x->d, x->u, x->v, x->w, x->T, x->P, x->amu, x->du, x->ev};
+}
+template <> View<float> view<float>(ocfd_ctx* x) {
+    return {x->sd, x->su, x->sv, x->sw, x->sT, x->sP, x->samu, x->sdu, x->sev};
+}
 
 hipEvent_t get_event(ocfd_ctx* x) {
     if (!x->pool.empty()) {
@@ -153,6 +174,24 @@ double* alloc_fields(ocfd_ctx* x, int n, int& rc) {
     return (double*)p + 12;
 }
 
+float* alloc_fields_f32(ocfd_ctx* x, int n, int& rc) {
+    // same element layout as alloc_fields; shifting the base by 28 elements
+    // puts the interior (x index 4) of every row on a 128-byte boundary (the
+    // mixed path rounds the row pitch to 32 elements)
+    size_t bytes = ((size_t)x->fs * n + 32) * sizeof(float);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        rc = fail(OCFD_ERR_HIP, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+        return nullptr;
+    }
+    x->allocs.push_back(p);
+    if (hipMemsetAsync(p, 0, bytes, x->st) != hipSuccess) {
+        rc = fail(OCFD_ERR_HIP, "hipMemsetAsync failed");
+        return nullptr;
+    }
+    return (float*)p + 28;
+}
+
 int8_t* alloc_bytes(ocfd_ctx* x, size_t n, int& rc) {
     void* p = nullptr;
     if (hipMalloc(&p, n + 256) != hipSuccess) {
@@ -195,23 +234,50 @@ int download_interior(ocfd_ctx* x, const double* src, double* dst) {
     return 0;
 }
 
-int fill_periodic(ocfd_ctx* x, double* base, int nf, const char* name) {
+template <typename T>
+int fill_periodic(ocfd_ctx* x, T* base, int nf, const char* name) {
     for (int dir = 0; dir < 3; dir++)
         if (x->cfg.iperiodic[dir])
             TRY(timed(x, name, [&] { launch_fill(x->g, base, nf, x->fs, dir, x->st); }));
     return 0;
 }
 
+template <typename S>
 int fill_prims(ocfd_ctx* x) {
     // d,u,v,w,T are allocated as one group of 5 contiguous fields
-    return fill_periodic(x, x->d, 5, "fill_prims");
+    return fill_periodic(x, view<S>(x).d, 5, "fill_prims");
 }
 
-BCArgs bc_args(ocfd_ctx* x) {
-    BCArgs a;
+// Field of any storage type <-> interior host array, through the double
+// scratch field for float fields.
+template <typename S>
+int upload_any(ocfd_ctx* x, S* dst, const double* src, double ghost_value) {
+    if constexpr (std::is_same<S, double>::value) {
+        return upload_interior(x, dst, src, ghost_value);
+    } else {
+        TRY(upload_interior(x, x->grad, src, ghost_value));
+        TRY(timed(x, "convert", [&] { launch_convert(dst, x->grad, x->fs, x->st); }));
+        HIPCK(hipStreamSynchronize(x->st));
+        return 0;
+    }
+}
+template <typename S>
+int download_any(ocfd_ctx* x, const S* src, double* dst) {
+    if constexpr (std::is_same<S, double>::value) {
+        return download_interior(x, src, dst);
+    } else {
+        TRY(timed(x, "convert", [&] { launch_convert(x->grad, src, x->fs, x->st); }));
+        return download_interior(x, x->grad, dst);
+    }
+}
+
+template <typename S>
+BCArgsT<S> bc_args(ocfd_ctx* x) {
+    BCArgsT<S> a;
+    const View<S> V = view<S>(x);
     a.g = x->g;
     a.c = x->c;
-    a.d = x->d; a.u = x->u; a.v = x->v; a.w = x->w; a.T = x->T; a.amu = x->amu; a.f = x->f;
+    a.d = V.d; a.u = V.u; a.v = V.v; a.w = V.w; a.T = V.T; a.amu = V.amu; a.f = x->f;
     a.fs = x->fs;
     a.ub1 = x->ub1; a.fx = x->fxd; a.gz = x->gzd;
     a.x_begin = x->x_begin;
@@ -224,8 +290,9 @@ BCArgs bc_args(ocfd_ctx* x) {
 
 // OCFD_bc (src/OCFD_boundary.cu:18-92) followed by the viscosity of the
 // faces it may change.
+template <typename S>
 int apply_bc(ocfd_ctx* x) {
-    BCArgs a = bc_args(x);
+    BCArgsT<S> a = bc_args<S>(x);
     if (x->cfg.ibc == 108) {
         double ht = 0.0;
         const double beta = x->cfg.bc_rpara[4];
@@ -243,22 +310,24 @@ int apply_bc(ocfd_ctx* x) {
     return 0;
 }
 
+template <typename S>
 int rhs(ocfd_ctx* x, int krk) {
     const ocfd_config& C = x->cfg;
+    const View<S> V = view<S>(x);
     if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1)
-        TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, x->d, x->T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
-    TRY(fill_prims(x));
+        TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
+    TRY(fill_prims<S>(x));
     static const char* names[3] = {"sweep_x", "sweep_y", "sweep_z"};
     const int mk[3][3] = {{M_KX, M_KY, M_KZ}, {M_IX, M_IY, M_IZ}, {M_SX, M_SY, M_SZ}};
     for (int dir = 0; dir < 3; dir++) {
-        SweepArgs a;
+        SweepArgsT<S> a;
         a.g = x->g;
         a.c = x->c;
-        a.d = x->d; a.u = x->u; a.v = x->v; a.w = x->w; a.T = x->T;
+        a.d = V.d; a.u = V.u; a.v = V.v; a.w = V.w; a.T = V.T;
         a.m1 = x->M[mk[dir][0]]; a.m2 = x->M[mk[dir][1]]; a.m3 = x->M[mk[dir][2]];
         a.jac = x->M[M_JAC];
         a.flag = x->flag[dir];
-        a.du = x->du;
+        a.du = V.du;
         a.fs = x->fs;
         a.h = dir == 0 ? x->g.hx : dir == 1 ? x->g.hy : x->g.hz;
         a.bl = C.dif_bound[2 * dir]; a.br = C.dif_bound[2 * dir + 1];
@@ -267,18 +336,18 @@ int rhs(ocfd_ctx* x, int krk) {
         TRY(timed(x, names[dir], [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
         if (lrc) return fail(OCFD_ERR_UNSUPPORTED, "scheme not supported");
     }
-    ViscArgs va;
+    ViscArgsT<S> va;
     va.g = x->g;
     va.c = x->c;
     va.svis = C.scheme_vis;
     for (int q = 0; q < 6; q++) va.bnd[q] = C.dif_bound[q];
-    va.u = x->u; va.v = x->v; va.w = x->w; va.T = x->T; va.amu = x->amu;
+    va.u = V.u; va.v = V.v; va.w = V.w; va.T = V.T; va.amu = V.amu;
     for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-    va.ev = x->ev; va.du = x->du; va.fs = x->fs;
+    va.ev = V.ev; va.du = V.du; va.fs = x->fs;
     TRY(timed(x, "visc_flux", [&] { launch_visc_gf(va, x->st); }));
     for (int dir = 0; dir < 3; dir++)
         if (C.iperiodic[dir])
-            TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, x->ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
+            TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
     TRY(timed(x, "visc_div", [&] { launch_visc_div(va, x->st); }));
     return 0;
 }
@@ -293,11 +362,13 @@ bool filter_due(ocfd_ctx* x, int istep) {
 // filter_{x,y,z}3d_shock (src/OCFD_filtering.cu:418-606): the x and y launches
 // cover ib <= i < ie-1 in their direction; the z launch runs the y kernel and
 // covers jb <= j < min(je-1, jb + 6*ceil((je-jb)/8)).
+template <typename S>
 int do_filter(ocfd_ctx* x) {
     if (!filter_due(x, x->istep)) return 0;
+    const View<S> V = view<S>(x);
     for (int dir = 0; dir < 3; dir++)
         if (x->filter_dir[dir] && x->cfg.iperiodic[dir]) {
-            TRY(timed(x, "fill_filter", [&] { launch_fill(x->g, x->P, 1, x->fs, dir, x->st); }));
+            TRY(timed(x, "fill_filter", [&] { launch_fill(x->g, V.P, 1, x->fs, dir, x->st); }));
             TRY(timed(x, "fill_filter", [&] { launch_fill(x->g, x->flap, 5, x->fs, dir, x->st); }));
         }
     for (int m = 0; m < x->cfg.nfilter; m++) {
@@ -305,8 +376,8 @@ int do_filter(ocfd_ctx* x) {
         const int* p = x->fpara[m];
         if (p[0] <= 0 || x->istep % p[0] != 0) continue;
         if (p[10] != 2) continue;
-        FilterArgs a;
-        a.g = x->g; a.P = x->P; a.flap = x->flap; a.f = x->f; a.fs = x->fs;
+        FilterArgsT<S> a;
+        a.g = x->g; a.P = V.P; a.flap = x->flap; a.f = x->f; a.fs = x->fs;
         a.s0 = x->cfg.filter_rpara[m][0];
         a.rth = x->cfg.filter_rpara[m][1];
         const int ib = p[4], ie = p[5], jb = p[6], je = p[7], kb = p[8], ke = p[9];
@@ -331,25 +402,39 @@ int do_filter(ocfd_ctx* x) {
     return 0;
 }
 
-int one_step(ocfd_ctx* x) {
+template <typename S>
+StageArgsT<S> stage_args(ocfd_ctx* x, int krk, bool write_flap) {
+    const View<S> V = view<S>(x);
+    StageArgsT<S> sa;
+    sa.g = x->g; sa.c = x->c; sa.krk = krk; sa.dt = x->cfg.dt;
+    sa.f = x->f; sa.fn = x->fn; sa.du = V.du;
+    sa.flap = write_flap ? x->flap : nullptr;
+    sa.d = V.d; sa.u = V.u; sa.v = V.v; sa.w = V.w; sa.T = V.T; sa.P = V.P; sa.amu = V.amu;
+    sa.fs = x->fs;
+    return sa;
+}
+
+template <typename S>
+int one_step_t(ocfd_ctx* x) {
     std::swap(x->f, x->fn);
     const bool fdue = x->flap && filter_due(x, x->istep + 1);
     for (int krk = 1; krk <= 3; krk++) {
-        TRY(rhs(x, krk));
-        StageArgs sa;
-        sa.g = x->g; sa.c = x->c; sa.krk = krk; sa.dt = x->cfg.dt;
-        sa.f = x->f; sa.fn = x->fn; sa.du = x->du;
-        sa.flap = (krk == 3 && fdue) ? x->flap : nullptr;
-        sa.d = x->d; sa.u = x->u; sa.v = x->v; sa.w = x->w; sa.T = x->T; sa.P = x->P; sa.amu = x->amu;
-        sa.fs = x->fs;
+        TRY(rhs<S>(x, krk));
+        const StageArgsT<S> sa = stage_args<S>(x, krk, krk == 3 && fdue);
         TRY(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
-        TRY(apply_bc(x));
+        TRY(apply_bc<S>(x));
     }
     x->istep++;
     x->tt += x->cfg.dt;
-    TRY(do_filter(x));
+    TRY(do_filter<S>(x));
     return 0;
 }
+
+// Precision dispatch of the templated drivers.
+int one_step(ocfd_ctx* x) { return x->mixed ? one_step_t<float>(x) : one_step_t<double>(x); }
+int rhs_any(ocfd_ctx* x, int krk) { return x->mixed ? rhs<float>(x, krk) : rhs<double>(x, krk); }
+int apply_bc_any(ocfd_ctx* x) { return x->mixed ? apply_bc<float>(x) : apply_bc<double>(x); }
+int do_filter_any(ocfd_ctx* x) { return x->mixed ? do_filter<float>(x) : do_filter<double>(x); }
 
 // IBC 108 set-up, bc_user_Compression_conner_init (src/OCFD_boundary_init.c:
 // 376-516), with glibc rand() from its default seed as in the reference.
@@ -423,6 +508,21 @@ int ibc108_init(ocfd_ctx* x) {
     return 0;
 }
 
+// ocfd_init body: pri_to_cons over the interior (src/OCFD_init.cu:99-110),
+// the IBC 108 set-up, then the NS_solver_real prologue
+// (src/OCFD_NS_Solver.cu:46-54): ghost fill, BC, viscosity everywhere.
+template <typename S>
+int init_fields(ocfd_ctx* x) {
+    BCArgsT<S> a = bc_args<S>(x);
+    TRY(timed(x, "prim2cons", [&] { launch_p2c_all(a, x->st); }));
+    if (x->cfg.ibc == 108) TRY(ibc108_init(x));
+    TRY(fill_prims<S>(x));
+    TRY(apply_bc<S>(x));
+    a = bc_args<S>(x);
+    TRY(timed(x, "amu", [&] { launch_amu_all(a, x->st); }));
+    return 0;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -446,7 +546,8 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     if (C.scheme_invis == OCFD_HYBRIDAUTO && C.hy_style != 2)
         return fail(OCFD_ERR_UNSUPPORTED, "hybrid_auto: only HY_STYLE = 2 (Jameson sensor) is supported");
     if (C.ibc != 0 && C.ibc != 108) return fail(OCFD_ERR_UNSUPPORTED, "IBC: only 0 and 108 are supported");
-    if (C.precision != OCFD_PREC_FP64) return fail(OCFD_ERR_UNSUPPORTED, "only the FP64 path is built");
+    if (C.precision != OCFD_PREC_FP64 && C.precision != OCFD_PREC_MIXED)
+        return fail(OCFD_ERR_CONFIG, "PRECISION: FP64 or MIXED");
     const long long npts = (long long)(C.nx + 8) * (C.ny + 8) * (C.nz + 8);
     if (npts > (1LL << 31)) return fail(OCFD_ERR_CONFIG, "grid too large for one context");
 
@@ -463,7 +564,10 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     }
     Geo& g = x->g;
     g.nx = C.nx; g.ny = C.ny; g.nz = C.nz;
-    g.ldx = ((C.nx + 8) + 15) / 16 * 16;
+    // row pitch: 128-byte rows in FP64; 32 elements (128 bytes) in FP32 too
+    // on the mixed path
+    const int pitch = C.precision == OCFD_PREC_MIXED ? 32 : 16;
+    g.ldx = ((C.nx + 8) + pitch - 1) / pitch * pitch;
     g.ldy = C.ny + 8;
     g.sz = (long long)g.ldx * g.ldy;
     g.off0 = 4 + (long long)g.ldx * (4 + (long long)g.ldy * 4);
@@ -471,7 +575,7 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     g.hx = 1.0 / (C.nx - 1.0);
     g.hy = 1.0 / (C.ny - 1.0);
     g.hz = 1.0 / (C.nz - 1.0);
-    x->fs = g.sz * (C.nz + 8) + 16;
+    x->fs = g.sz * (C.nz + 8) + 32;  // a multiple of 32 elements: every field of a group stays aligned
     // opencfd_para_init, src/OCFD_init.cu:381-410
     Consts& c = x->c;
     const double gm = C.gamma, Ma = C.ama;
@@ -487,30 +591,47 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     c.vis_c = c.cp / C.pr;
     c.epsl_sw = C.epsl_sw;
     c.p00 = 1.0 / (gm * Ma * Ma);
+    c.rama = 1.0 / Ma;
     c.hy_i1 = C.hy_dp_intv[0];
     c.hy_i2 = C.hy_dp_intv[1];
     c.tv_lim = 5.0;  // src/parameters.c:75-76
     c.tv_max = 0.2;
 
+    x->mixed = C.precision == OCFD_PREC_MIXED;
     int rc = 0;
-    double* prim = alloc_fields(x, 6, rc);  // d,u,v,w,T,P
-    double* met = rc ? nullptr : alloc_fields(x, M_COUNT, rc);
+    double* met = alloc_fields(x, M_COUNT, rc);
     double* fa = rc ? nullptr : alloc_fields(x, 5, rc);
     double* fb = rc ? nullptr : alloc_fields(x, 5, rc);
-    double* dua = rc ? nullptr : alloc_fields(x, 5, rc);
-    double* am = rc ? nullptr : alloc_fields(x, 1, rc);
     double* gr = rc ? nullptr : alloc_fields(x, 12, rc);
-    double* ev = rc ? nullptr : alloc_fields(x, 12, rc);
     if (!rc && C.nfilter > 0) x->flap = alloc_fields(x, 5, rc);
+    if (!x->mixed) {
+        double* prim = rc ? nullptr : alloc_fields(x, 6, rc);  // d,u,v,w,T,P
+        double* dua = rc ? nullptr : alloc_fields(x, 5, rc);
+        double* am = rc ? nullptr : alloc_fields(x, 1, rc);
+        double* ev = rc ? nullptr : alloc_fields(x, 12, rc);
+        if (!rc) {
+            x->d = prim; x->u = prim + x->fs; x->v = prim + 2 * x->fs; x->w = prim + 3 * x->fs;
+            x->T = prim + 4 * x->fs; x->P = prim + 5 * x->fs;
+            x->du = dua; x->amu = am; x->ev = ev;
+        }
+    } else {
+        float* prim = rc ? nullptr : alloc_fields_f32(x, 6, rc);
+        float* dua = rc ? nullptr : alloc_fields_f32(x, 5, rc);
+        float* am = rc ? nullptr : alloc_fields_f32(x, 1, rc);
+        float* ev = rc ? nullptr : alloc_fields_f32(x, 12, rc);
+        if (!rc) {
+            x->sd = prim; x->su = prim + x->fs; x->sv = prim + 2 * x->fs; x->sw = prim + 3 * x->fs;
+            x->sT = prim + 4 * x->fs; x->sP = prim + 5 * x->fs;
+            x->sdu = dua; x->samu = am; x->sev = ev;
+        }
+    }
     for (int dd = 0; dd < 3 && !rc; dd++) x->flag[dd] = alloc_bytes(x, (size_t)x->fs, rc);
     if (rc) {
         ocfd_destroy(x);
         return rc;
     }
-    x->d = prim; x->u = prim + x->fs; x->v = prim + 2 * x->fs; x->w = prim + 3 * x->fs; x->T = prim + 4 * x->fs;
-    x->P = prim + 5 * x->fs;
     for (int m = 0; m < M_COUNT; m++) x->M[m] = met + m * x->fs;
-    x->f = fa; x->fn = fb; x->du = dua; x->amu = am; x->grad = gr; x->ev = ev;
+    x->f = fa; x->fn = fb; x->grad = gr;
     // metrics start at 1.0 everywhere (src/OCFD_Comput_Jacobian3d.cu:34-48)
     launch_set(g, met - 12, x->fs * M_COUNT + 16, 1.0, x->st);
     // set_para_filtering clamps at one rank (src/OCFD_filtering.cu:20-73)
@@ -605,8 +726,13 @@ int ocfd_load_flow(ocfd_ctx* x, const double* d, const double* u, const double* 
                    const double* T, int istep, double tt) {
     if (!x || !d || !u || !v || !w || !T) return fail(OCFD_ERR_ARG, "null argument");
     const double* src[5] = {d, u, v, w, T};
-    double* dst[5] = {x->d, x->u, x->v, x->w, x->T};
-    for (int q = 0; q < 5; q++) TRY(upload_interior(x, dst[q], src[q], 0.0));
+    if (x->mixed) {
+        float* dst[5] = {x->sd, x->su, x->sv, x->sw, x->sT};
+        for (int q = 0; q < 5; q++) TRY(upload_any(x, dst[q], src[q], 0.0));
+    } else {
+        double* dst[5] = {x->d, x->u, x->v, x->w, x->T};
+        for (int q = 0; q < 5; q++) TRY(upload_any(x, dst[q], src[q], 0.0));
+    }
     x->istep = istep;
     x->tt = tt;
     x->have_flow = true;
@@ -623,16 +749,7 @@ int ocfd_set_inlet_profile(ocfd_ctx* x, const double* prof) {
 int ocfd_init(ocfd_ctx* x) {
     if (!x) return fail(OCFD_ERR_ARG, "null context");
     if (!x->have_grid || !x->have_flow) return fail(OCFD_ERR_STATE, "load the grid and the flow before ocfd_init");
-    // pri_to_cons over the interior (src/OCFD_init.cu:99-110)
-    BCArgs a = bc_args(x);
-    TRY(timed(x, "prim2cons", [&] { launch_p2c_all(a, x->st); }));
-    if (x->cfg.ibc == 108) TRY(ibc108_init(x));
-    // NS_solver_real prologue (src/OCFD_NS_Solver.cu:46-54): ghost fill, BC,
-    // viscosity everywhere
-    TRY(fill_prims(x));
-    TRY(apply_bc(x));
-    a = bc_args(x);
-    TRY(timed(x, "amu", [&] { launch_amu_all(a, x->st); }));
+    TRY(x->mixed ? init_fields<float>(x) : init_fields<double>(x));
     HIPCK(hipStreamSynchronize(x->st));
     x->inited = true;
     return 0;
@@ -640,7 +757,7 @@ int ocfd_init(ocfd_ctx* x) {
 
 int ocfd_rhs(ocfd_ctx* x, int krk) {
     if (!x || !x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
-    TRY(rhs(x, krk));
+    TRY(rhs_any(x, krk));
     TRY(flush_prof(x));
     return 0;
 }
@@ -669,9 +786,21 @@ double ocfd_last_step_ms(ocfd_ctx* x) { return x ? x->last_step_ms : 0.0; }
 int ocfd_get_flow(ocfd_ctx* x, double* d, double* u, double* v, double* w, double* T, int* istep, double* tt) {
     if (!x) return fail(OCFD_ERR_ARG, "null context");
     double* dst[5] = {d, u, v, w, T};
-    const double* src[5] = {x->d, x->u, x->v, x->w, x->T};
-    for (int q = 0; q < 5; q++)
-        if (dst[q]) TRY(download_interior(x, src[q], dst[q]));
+    if (x->mixed) {
+        // FP64 primitives from the FP64 conserved state (get_duvwT arithmetic)
+        // into the scratch fields
+        StageArgsT<double> sa;
+        sa.g = x->g; sa.c = x->c; sa.f = x->f; sa.fs = x->fs;
+        sa.d = x->grad; sa.u = x->grad + x->fs; sa.v = x->grad + 2 * x->fs; sa.w = x->grad + 3 * x->fs;
+        sa.T = x->grad + 4 * x->fs; sa.P = x->grad + 5 * x->fs;
+        TRY(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
+        for (int q = 0; q < 5; q++)
+            if (dst[q]) TRY(download_interior(x, x->grad + q * x->fs, dst[q]));
+    } else {
+        const double* src[5] = {x->d, x->u, x->v, x->w, x->T};
+        for (int q = 0; q < 5; q++)
+            if (dst[q]) TRY(download_interior(x, src[q], dst[q]));
+    }
     if (istep) *istep = x->istep;
     if (tt) *tt = x->tt;
     return 0;
@@ -687,18 +816,20 @@ int ocfd_get_state(ocfd_ctx* x, double* f5) {
 int ocfd_get_du(ocfd_ctx* x, double* du5) {
     if (!x || !du5) return fail(OCFD_ERR_ARG, "null argument");
     const size_t n = (size_t)x->g.nx * x->g.ny * x->g.nz;
-    for (int m = 0; m < 5; m++) TRY(download_interior(x, x->du + m * x->fs, du5 + m * n));
+    for (int m = 0; m < 5; m++)
+        TRY(x->mixed ? download_any(x, x->sdu + m * x->fs, du5 + m * n)
+                     : download_any(x, x->du + m * x->fs, du5 + m * n));
     return 0;
 }
 
 int ocfd_get_pressure(ocfd_ctx* x, double* p) {
     if (!x || !p) return fail(OCFD_ERR_ARG, "null argument");
-    return download_interior(x, x->P, p);
+    return x->mixed ? download_any(x, x->sP, p) : download_any(x, x->P, p);
 }
 
 int ocfd_get_amu(ocfd_ctx* x, double* a) {
     if (!x || !a) return fail(OCFD_ERR_ARG, "null argument");
-    return download_interior(x, x->amu, a);
+    return x->mixed ? download_any(x, x->samu, a) : download_any(x, x->amu, a);
 }
 
 int ocfd_get_metrics(ocfd_ctx* x, double* a13) {
@@ -894,37 +1025,47 @@ void NS_solver_real(void) {
     std::printf("OK! opencfd is finished\nTotal GPU time %lf\n", total * 1e-3);
 }
 
-void du_comput(int KRK) { SHIM(rhs(g_ctx, KRK)); }
+void du_comput(int KRK) { SHIM(rhs_any(g_ctx, KRK)); }
 
 void OCFD_time_advance(int KRK) {
-    StageArgs sa;
     ocfd_ctx* x = g_ctx;
-    sa.g = x->g; sa.c = x->c; sa.krk = KRK; sa.dt = x->cfg.dt;
-    sa.f = x->f; sa.fn = x->fn; sa.du = x->du;
-    sa.flap = (KRK == 3 && x->flap && filter_due(x, x->istep + 1)) ? x->flap : nullptr;
-    sa.d = x->d; sa.u = x->u; sa.v = x->v; sa.w = x->w; sa.T = x->T; sa.P = x->P; sa.amu = x->amu;
-    sa.fs = x->fs;
-    SHIM(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
+    const bool flap = KRK == 3 && x->flap && filter_due(x, x->istep + 1);
+    if (x->mixed) {
+        const StageArgsT<float> sa = stage_args<float>(x, KRK, flap);
+        SHIM(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
+    } else {
+        const StageArgsT<double> sa = stage_args<double>(x, KRK, flap);
+        SHIM(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
+    }
 }
 
 void get_duvwT(void) {
-    StageArgs sa;
     ocfd_ctx* x = g_ctx;
-    sa.g = x->g; sa.c = x->c; sa.f = x->f; sa.d = x->d; sa.u = x->u; sa.v = x->v; sa.w = x->w; sa.T = x->T;
-    sa.P = x->P; sa.fs = x->fs;
-    SHIM(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
+    if (x->mixed) {
+        const StageArgsT<float> sa = stage_args<float>(x, 1, false);
+        SHIM(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
+    } else {
+        const StageArgsT<double> sa = stage_args<double>(x, 1, false);
+        SHIM(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
+    }
 }
 
 void get_Amu(void) {
-    BCArgs a = bc_args(g_ctx);
-    SHIM(timed(g_ctx, "amu", [&] { launch_amu_all(a, g_ctx->st); }));
+    ocfd_ctx* x = g_ctx;
+    if (x->mixed) {
+        const BCArgsT<float> a = bc_args<float>(x);
+        SHIM(timed(x, "amu", [&] { launch_amu_all(a, x->st); }));
+    } else {
+        const BCArgsT<double> a = bc_args<double>(x);
+        SHIM(timed(x, "amu", [&] { launch_amu_all(a, x->st); }));
+    }
 }
 
-void OCFD_bc(void) { SHIM(apply_bc(g_ctx)); }
+void OCFD_bc(void) { SHIM(apply_bc_any(g_ctx)); }
 
 void filtering(double* pf, double* pf0, double* pp) {
     (void)pf; (void)pf0; (void)pp;
-    SHIM(do_filter(g_ctx));
+    SHIM(do_filter_any(g_ctx));
 }
 
 void opencfd_mem_finalize_all(void) {

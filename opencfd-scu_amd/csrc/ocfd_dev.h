@@ -33,6 +33,7 @@ enum { M_KX = 0, M_KY, M_KZ, M_IX, M_IY, M_IZ, M_SX, M_SY, M_SZ, M_JAC, M_COUNT 
 struct Consts {
     double gamma, ama, cv, cp, tsb, amu_c0, tmp0, split_c1, split_c3, vis_c;
     double epsl_sw, p00;                     // p00 = 1/(gamma*Ma^2)
+    double rama;                             // 1/Ma
     double hy_i1, hy_i2;                     // HY_DP_INTV
     double tv_lim, tv_max;                   // WENO_TV_Limiter / WENO_TV_MAX
 };
@@ -54,7 +55,9 @@ template <typename R> __device__ __forceinline__ R d_minmod4(R a, R b, R c, R d)
 }
 
 // WENO7 upwind (P) half-node value from nodes s[0..6] (src/OCFD_Schemes.cu:616-663).
-template <typename R> __device__ __forceinline__ R weno7_up(const R* s) {
+// R: type of the stencil values and smoothness indicators; W: type of the
+// epsilon/weight arithmetic (the mixed-precision path keeps W = double).
+template <typename R, typename W = R> __device__ __forceinline__ R weno7_up(const R* s) {
     R a, b, c, S0, S1, S2, S3;
     a = R(-2) * s[0] + R(9) * s[1] - R(18) * s[2] + R(11) * s[3];
     b = R(-6) * s[0] + R(24) * s[1] - R(30) * s[2] + R(12) * s[3];
@@ -72,34 +75,37 @@ template <typename R> __device__ __forceinline__ R weno7_up(const R* s) {
     b = R(12) * s[3] - R(30) * s[4] + R(24) * s[5] - R(6) * s[6];
     c = R(-6) * s[3] + R(18) * (s[4] - s[5]) + R(6) * s[6];
     S3 = R(960) * a * a; S3 += R(1040) * b * b; S3 += R(1043) * c * c; S3 += R(80) * (a * c);
-    const R e = R(3.456e-4);
-    R q0 = (e + S0) * (e + S0), q1 = (e + S1) * (e + S1), q2 = (e + S2) * (e + S2), q3 = (e + S3) * (e + S3);
-    R w0 = q1 * q2 * q3;
-    R w1 = R(12) * q0 * q2 * q3;
-    R w2 = R(18) * q0 * q1 * q3;
-    R w3 = R(4) * q0 * q1 * q2;
-    R am = w0 + w1 + w2 + w3;
-    R r = w0 * (R(-3) * s[0] + R(13) * s[1] - R(23) * s[2] + R(25) * s[3]);
-    r += w1 * (s[1] - R(5) * s[2] + R(13) * s[3] + R(3) * s[4]);
-    r += w2 * (-s[2] + R(7) * s[3] + R(7) * s[4] - s[5]);
-    r += w3 * (R(3) * s[3] + R(13) * s[4] - R(5) * s[5] + s[6]);
-    return r / (R(12) * am);
+    const W e = W(3.456e-4);
+    W q0 = (e + W(S0)) * (e + W(S0)), q1 = (e + W(S1)) * (e + W(S1)), q2 = (e + W(S2)) * (e + W(S2)),
+      q3 = (e + W(S3)) * (e + W(S3));
+    W w0 = q1 * q2 * q3;
+    W w1 = W(12) * q0 * q2 * q3;
+    W w2 = W(18) * q0 * q1 * q3;
+    W w3 = W(4) * q0 * q1 * q2;
+    W am = w0 + w1 + w2 + w3;
+    W r = w0 * W(R(-3) * s[0] + R(13) * s[1] - R(23) * s[2] + R(25) * s[3]);
+    r += w1 * W(s[1] - R(5) * s[2] + R(13) * s[3] + R(3) * s[4]);
+    r += w2 * W(-s[2] + R(7) * s[3] + R(7) * s[4] - s[5]);
+    r += w3 * W(R(3) * s[3] + R(13) * s[4] - R(5) * s[5] + s[6]);
+    return R(r / (W(12) * am));
 }
 // WENO7 M side: the same reconstruction on the mirrored stencil
 // (src/OCFD_Schemes.cu:692-741); s[0..6].
-template <typename R> __device__ __forceinline__ R weno7_dn(const R* s) {
+template <typename R, typename W = R> __device__ __forceinline__ R weno7_dn(const R* s) {
     R m[7] = {s[6], s[5], s[4], s[3], s[2], s[1], s[0]};
-    return weno7_up(m);
+    return weno7_up<R, W>(m);
 }
 
-// UP7 (src/OCFD_Schemes.cu:865-931)
+// UP7 (src/OCFD_Schemes.cu:865-931).  The division by 420 is a multiplication
+// by its reciprocal (an ulp-level difference; a division costs ten FP64
+// instructions, the whole reconstruction eight).
 template <typename R> __device__ __forceinline__ R up7_up(const R* s) {
     return (R(-3) * s[0] + R(25) * s[1] - R(101) * s[2] + R(319) * s[3] + R(214) * s[4] - R(38) * s[5] +
-            R(4) * s[6]) / R(420);
+            R(4) * s[6]) * R(1.0 / 420.0);
 }
 template <typename R> __device__ __forceinline__ R up7_dn(const R* s) {  // s[1..7]
     return (R(-3) * s[7] + R(25) * s[6] - R(101) * s[5] + R(319) * s[4] + R(214) * s[3] - R(38) * s[2] +
-            R(4) * s[1]) / R(420);
+            R(4) * s[1]) * R(1.0 / 420.0);
 }
 
 // WENO5 Jiang-Shu (src/OCFD_Schemes.cu:1118-1210) and the one-sided subsets
@@ -116,103 +122,103 @@ template <typename R> __device__ __forceinline__ R w5_beta_r(const R* s) {  // s
     R t = s[2] - R(2) * s[3] + s[4], u = R(3) * s[2] - R(4) * s[3] + s[4];
     R S = R(13) * t * t; S += R(3) * u * u; return S;
 }
-template <typename R> __device__ __forceinline__ R weno5_up(const R* s) {
-    const R e = R(12.0) * R(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
-    R S0 = w5_beta_r(s), S1 = w5_beta_c(s), S2 = w5_beta_l(s);
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_up(const R* s) {
+    const W e = W(12.0) * W(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
+    const W S0 = w5_beta_r(s), S1 = w5_beta_c(s), S2 = w5_beta_l(s);
     R q03 = R(2) * s[2] + R(5) * s[3] - s[4];
     R q13 = -s[1] + R(5) * s[2] + R(2) * s[3];
     R q23 = R(2) * s[0] - R(7) * s[1] + R(11) * s[2];
-    R a0 = R(3) * ((e + S1) * (e + S1)) * ((e + S2) * (e + S2));
-    R a1 = R(6) * ((e + S0) * (e + S0)) * ((e + S2) * (e + S2));
-    R a2 = ((e + S0) * (e + S0)) * ((e + S1) * (e + S1));
-    return (a0 * q03 + a1 * q13 + a2 * q23) / (R(6) * (a0 + a1 + a2));
+    W a0 = W(3) * ((e + S1) * (e + S1)) * ((e + S2) * (e + S2));
+    W a1 = W(6) * ((e + S0) * (e + S0)) * ((e + S2) * (e + S2));
+    W a2 = ((e + S0) * (e + S0)) * ((e + S1) * (e + S1));
+    return R((a0 * W(q03) + a1 * W(q13) + a2 * W(q23)) / (W(6) * (a0 + a1 + a2)));
 }
-template <typename R> __device__ __forceinline__ R weno5_dn(const R* s) {
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_dn(const R* s) {
     R m[5] = {s[4], s[3], s[2], s[1], s[0]};
-    return weno5_up(m);
+    return weno5_up<R, W>(m);
 }
-template <typename R> __device__ __forceinline__ R weno5_up_left(const R* s) {   // OCFD_weno5_kernel_P_left
-    const R e = R(12.0) * R(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
-    R S0 = w5_beta_r(s);
-    R a0 = R(1) / ((e + S0) * (e + S0));
-    return a0 * (R(2) * s[2] + R(5) * s[3] - s[4]) / (R(6) * a0);
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_up_left(const R* s) {  // OCFD_weno5_kernel_P_left
+    const W e = W(12.0) * W(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
+    const W S0 = w5_beta_r(s);
+    W a0 = W(1) / ((e + S0) * (e + S0));
+    return R(a0 * W(R(2) * s[2] + R(5) * s[3] - s[4]) / (W(6) * a0));
 }
-template <typename R> __device__ __forceinline__ R weno5_up_leftplus(const R* s) {
-    const R e = R(12.0) * R(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
-    R S0 = w5_beta_r(s), S1 = w5_beta_c(s);
-    R a0 = R(3) / ((e + S0) * (e + S0));
-    R a1 = R(6) / ((e + S1) * (e + S1));
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_up_leftplus(const R* s) {
+    const W e = W(12.0) * W(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
+    const W S0 = w5_beta_r(s), S1 = w5_beta_c(s);
+    W a0 = W(3) / ((e + S0) * (e + S0));
+    W a1 = W(6) / ((e + S1) * (e + S1));
     R q03 = R(2) * s[2] + R(5) * s[3] - s[4];
     R q13 = -s[1] + R(5) * s[2] + R(2) * s[3];
-    return (a0 * q03 + a1 * q13) / (R(6) * (a0 + a1));
+    return R((a0 * W(q03) + a1 * W(q13)) / (W(6) * (a0 + a1)));
 }
-template <typename R> __device__ __forceinline__ R weno5_up_rightplus(const R* s) {
-    const R e = R(12.0) * R(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
-    R S1 = w5_beta_c(s), S2 = w5_beta_l(s);
-    R a1 = R(6) / ((e + S1) * (e + S1));
-    R a2 = R(1) / ((e + S2) * (e + S2));
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_up_rightplus(const R* s) {
+    const W e = W(12.0) * W(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
+    const W S1 = w5_beta_c(s), S2 = w5_beta_l(s);
+    W a1 = W(6) / ((e + S1) * (e + S1));
+    W a2 = W(1) / ((e + S2) * (e + S2));
     R q13 = -s[1] + R(5) * s[2] + R(2) * s[3];
     R q23 = R(2) * s[0] - R(7) * s[1] + R(11) * s[2];
-    return (a1 * q13 + a2 * q23) / (R(6) * (a1 + a2));
+    return R((a1 * W(q13) + a2 * W(q23)) / (W(6) * (a1 + a2)));
 }
-template <typename R> __device__ __forceinline__ R weno5_dn_left(const R* s) {   // OCFD_weno5_kernel_M_left
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_dn_left(const R* s) {  // OCFD_weno5_kernel_M_left
     R m[5] = {s[4], s[3], s[2], s[1], s[0]};
-    const R e = R(12.0) * R(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
-    R S2 = w5_beta_l(m);
-    R a2 = R(1) / ((e + S2) * (e + S2));
-    return a2 * (R(2) * m[0] - R(7) * m[1] + R(11) * m[2]) / (R(6) * a2);
+    const W e = W(12.0) * W(1e-6);  // 12.0*ep with ep = 1e-6, as the reference writes it
+    const W S2 = w5_beta_l(m);
+    W a2 = W(1) / ((e + S2) * (e + S2));
+    return R(a2 * W(R(2) * m[0] - R(7) * m[1] + R(11) * m[2]) / (W(6) * a2));
 }
-template <typename R> __device__ __forceinline__ R weno5_dn_leftplus(const R* s) {
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_dn_leftplus(const R* s) {
     R m[5] = {s[4], s[3], s[2], s[1], s[0]};
-    return weno5_up_rightplus(m);
+    return weno5_up_rightplus<R, W>(m);
 }
-template <typename R> __device__ __forceinline__ R weno5_dn_rightplus(const R* s) {
+template <typename R, typename W = R> __device__ __forceinline__ R weno5_dn_rightplus(const R* s) {
     R m[5] = {s[4], s[3], s[2], s[1], s[0]};
-    return weno5_up_leftplus(m);
+    return weno5_up_leftplus<R, W>(m);
 }
 
 // WENO7-SYMBO (src/OCFD_Schemes.cu:391-589); P side with optional TV
 // limiter; the M side never takes the linear branch (TVmin stays 0).
-template <typename R> __device__ __forceinline__ R symbo_core(const R* s, bool linear) {
-    R S0, S1, S2, S3, S4, t;
+template <typename R, typename W = R> __device__ __forceinline__ R symbo_core(const R* s, bool linear) {
+    W S0, S1, S2, S3, S4;
     if (linear) {
-        S0 = R(0.0401954833730); S1 = R(0.2493800006710); S2 = R(0.4802686256260);
-        S3 = R(0.2009775476730); S4 = R(0.0291783426580);
+        S0 = W(0.0401954833730); S1 = W(0.2493800006710); S2 = W(0.4802686256260);
+        S3 = W(0.2009775476730); S4 = W(0.0291783426580);
     } else {
-        t = R(-2) * s[0] + R(9) * s[1] - R(18) * s[2] + R(11) * s[3]; S0 = R(720) * t * t;
-        t = s[1] - R(6) * s[2] + R(3) * s[3] + R(2) * s[4]; S1 = R(720) * t * t;
-        t = R(-2) * s[2] - R(3) * s[3] + R(6) * s[4] - s[5]; S2 = R(720) * t * t;
-        t = R(-11) * s[3] + R(18) * s[4] - R(9) * s[5] + R(2) * s[6]; S3 = R(720) * t * t;
-        t = R(-26) * s[4] + R(57) * s[5] - R(42) * s[6] + R(11) * s[7]; S4 = R(720) * t * t;
-        t = R(-6) * s[0] + R(24) * s[1] - R(30) * s[2] + R(12) * s[3]; S0 += R(780) * t * t;
-        t = R(6) * s[2] - R(12) * s[3] + R(6) * s[4]; S1 += R(780) * t * t;
-        t = R(6) * s[3] - R(12) * s[4] + R(6) * s[5]; S2 += R(780) * t * t;
-        t = R(12) * s[3] - R(30) * s[4] + R(24) * s[5] - R(6) * s[6]; S3 += R(780) * t * t;
-        t = R(18) * s[4] - R(48) * s[5] + R(42) * s[6] - R(12) * s[7]; S4 += R(780) * t * t;
-        t = R(-6) * s[0] + R(18) * (s[1] - s[2]) + R(6) * s[3]; S0 += R(781) * t * t;
-        t = R(-6) * s[1] + R(18) * (s[2] - s[3]) + R(6) * s[4]; S1 += R(781) * t * t;
-        t = R(-6) * s[2] + R(18) * (s[3] - s[4]) + R(6) * s[5]; S2 += R(781) * t * t;
-        t = R(-6) * s[3] + R(18) * (s[4] - s[5]) + R(6) * s[6]; S3 += R(781) * t * t;
-        t = R(-6) * s[4] + R(18) * (s[5] - s[6]) + R(6) * s[7]; S4 += R(781) * t * t;
-        S4 = fmax(fmax(fmax(S0, S1), fmax(S2, S3)), S4);
-        const R e = R(2.592e-6);
-        R e0 = e + S0, e1 = e + S1, e2 = e + S2, e3 = e + S3, e4 = e + S4;
-        R w0 = R(0.0401954833730) * e1 * e2 * e3 * e4;
-        R w1 = R(0.2493800006710) * e0 * e2 * e3 * e4;
-        R w2 = R(0.4802686256260) * e0 * e1 * e3 * e4;
-        R w3 = R(0.2009775476730) * e0 * e1 * e2 * e4;
-        R w4 = R(0.0291783426580) * e0 * e1 * e2 * e3;
-        S0 = w0; S1 = w1; S2 = w2; S3 = w3; S4 = w4;
+        R t, B0, B1, B2, B3, B4;
+        t = R(-2) * s[0] + R(9) * s[1] - R(18) * s[2] + R(11) * s[3]; B0 = R(720) * t * t;
+        t = s[1] - R(6) * s[2] + R(3) * s[3] + R(2) * s[4]; B1 = R(720) * t * t;
+        t = R(-2) * s[2] - R(3) * s[3] + R(6) * s[4] - s[5]; B2 = R(720) * t * t;
+        t = R(-11) * s[3] + R(18) * s[4] - R(9) * s[5] + R(2) * s[6]; B3 = R(720) * t * t;
+        t = R(-26) * s[4] + R(57) * s[5] - R(42) * s[6] + R(11) * s[7]; B4 = R(720) * t * t;
+        t = R(-6) * s[0] + R(24) * s[1] - R(30) * s[2] + R(12) * s[3]; B0 += R(780) * t * t;
+        t = R(6) * s[2] - R(12) * s[3] + R(6) * s[4]; B1 += R(780) * t * t;
+        t = R(6) * s[3] - R(12) * s[4] + R(6) * s[5]; B2 += R(780) * t * t;
+        t = R(12) * s[3] - R(30) * s[4] + R(24) * s[5] - R(6) * s[6]; B3 += R(780) * t * t;
+        t = R(18) * s[4] - R(48) * s[5] + R(42) * s[6] - R(12) * s[7]; B4 += R(780) * t * t;
+        t = R(-6) * s[0] + R(18) * (s[1] - s[2]) + R(6) * s[3]; B0 += R(781) * t * t;
+        t = R(-6) * s[1] + R(18) * (s[2] - s[3]) + R(6) * s[4]; B1 += R(781) * t * t;
+        t = R(-6) * s[2] + R(18) * (s[3] - s[4]) + R(6) * s[5]; B2 += R(781) * t * t;
+        t = R(-6) * s[3] + R(18) * (s[4] - s[5]) + R(6) * s[6]; B3 += R(781) * t * t;
+        t = R(-6) * s[4] + R(18) * (s[5] - s[6]) + R(6) * s[7]; B4 += R(781) * t * t;
+        B4 = fmax(fmax(fmax(B0, B1), fmax(B2, B3)), B4);
+        const W e = W(2.592e-6);
+        W e0 = e + W(B0), e1 = e + W(B1), e2 = e + W(B2), e3 = e + W(B3), e4 = e + W(B4);
+        S0 = W(0.0401954833730) * e1 * e2 * e3 * e4;
+        S1 = W(0.2493800006710) * e0 * e2 * e3 * e4;
+        S2 = W(0.4802686256260) * e0 * e1 * e3 * e4;
+        S3 = W(0.2009775476730) * e0 * e1 * e2 * e4;
+        S4 = W(0.0291783426580) * e0 * e1 * e2 * e3;
     }
-    R am = S0 + S1 + S2 + S3 + S4;
-    R r = S0 * (R(-3) * s[0] + R(13) * s[1] - R(23) * s[2] + R(25) * s[3]);
-    r += S1 * (s[1] - R(5) * s[2] + R(13) * s[3] + R(3) * s[4]);
-    r += S2 * (-s[2] + R(7) * s[3] + R(7) * s[4] - s[5]);
-    r += S3 * (R(3) * s[3] + R(13) * s[4] - R(5) * s[5] + s[6]);
-    r += S4 * (R(25) * s[4] - R(23) * s[5] + R(13) * s[6] - R(3) * s[7]);
-    return r / (R(12) * am);
+    W am = S0 + S1 + S2 + S3 + S4;
+    W r = S0 * W(R(-3) * s[0] + R(13) * s[1] - R(23) * s[2] + R(25) * s[3]);
+    r += S1 * W(s[1] - R(5) * s[2] + R(13) * s[3] + R(3) * s[4]);
+    r += S2 * W(-s[2] + R(7) * s[3] + R(7) * s[4] - s[5]);
+    r += S3 * W(R(3) * s[3] + R(13) * s[4] - R(5) * s[5] + s[6]);
+    r += S4 * W(R(25) * s[4] - R(23) * s[5] + R(13) * s[6] - R(3) * s[7]);
+    return R(r / (W(12) * am));
 }
-template <typename R> __device__ __forceinline__ R symbo_up(const R* s, int lmt, R tvlim, R tvmax) {
+template <typename R, typename W = R> __device__ __forceinline__ R symbo_up(const R* s, int lmt, R tvlim, R tvmax) {
     bool lin = false;
     if (lmt) {
         R a0 = fabs(s[1] - s[0]) + fabs(s[2] - s[1]) + fabs(s[3] - s[2]);
@@ -226,11 +232,11 @@ template <typename R> __device__ __forceinline__ R symbo_up(const R* s, int lmt,
     } else {
         lin = (R(1) < tvlim * R(0)) && (R(1) < tvmax);
     }
-    return symbo_core(s, lin);
+    return symbo_core<R, W>(s, lin);
 }
-template <typename R> __device__ __forceinline__ R symbo_dn(const R* s) {
+template <typename R, typename W = R> __device__ __forceinline__ R symbo_dn(const R* s) {
     R m[8] = {s[7], s[6], s[5], s[4], s[3], s[2], s[1], s[0]};
-    return symbo_core(m, false);
+    return symbo_core<R, W>(m, false);
 }
 
 // OMP6 (src/OCFD_Schemes.cu:936-1092).  The M limiter is kept as in the

@@ -1542,6 +1542,7 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 int orc_get_state(orc *o, double *f5) { memcpy(f5, o->f, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_du(orc *o, double *du5) { memcpy(du5, o->du, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_pressure(orc *o, double *p) { get_interior(o, o->P, p); return 0; }
+int orc_get_pressure_lap(orc *o, double *p) { memcpy(p, o->P, sizeof(double) * o->nlap); return 0; }
 int orc_get_amu(orc *o, double *amu) { memcpy(amu, o->Amu, sizeof(double) * o->nint); return 0; }
 int orc_get_metrics(orc *o, double *a13) {
     for (int k = 0; k < 13; k++) get_interior(o, o->A[k], a13 + (long)k * o->nint);

@@ -41,6 +41,8 @@ int orc_get_flow(orc *o, double *d, double *u, double *v, double *w,
 int orc_get_state(orc *o, double *f5);
 int orc_get_du(orc *o, double *du5);
 int orc_get_pressure(orc *o, double *p);
+/* The pressure on the whole LAP box ((nz+8)(ny+8)(nx+8), x fastest). */
+int orc_get_pressure_lap(orc *o, double *p);
 int orc_get_amu(orc *o, double *amu);
 int orc_get_metrics(orc *o, double *a13);
 int orc_get_flags(orc *o, signed char *fx, signed char *fy, signed char *fz);

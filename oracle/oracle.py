@@ -55,6 +55,7 @@ def lib() -> C.CDLL:
             "orc_get_state": (C.c_int, [vp, dp]),
             "orc_get_du": (C.c_int, [vp, dp]),
             "orc_get_pressure": (C.c_int, [vp, dp]),
+            "orc_get_pressure_lap": (C.c_int, [vp, dp]),
             "orc_get_amu": (C.c_int, [vp, dp]),
             "orc_get_metrics": (C.c_int, [vp, dp]),
             "orc_get_flags": (C.c_int, [vp, i8, i8, i8]),
@@ -145,6 +146,15 @@ class Oracle:
         out = np.empty(self.nx * self.ny * self.nz)
         self.L.orc_get_pressure(self.h, out)
         return out.reshape(self.shape)
+
+    def get_pressure_lap(self, ghosts=1):
+        """Pressure with `ghosts` ghost layers on each side (at most 4)."""
+        n2 = (self.nx + 8) * (self.ny + 8) * (self.nz + 8)
+        out = np.empty(n2)
+        self.L.orc_get_pressure_lap(self.h, out)
+        out = out.reshape(self.nz + 8, self.ny + 8, self.nx + 8)
+        c = 4 - ghosts
+        return out[c:out.shape[0] - c, c:out.shape[1] - c, c:out.shape[2] - c]
 
     def get_amu(self):
         out = np.empty(self.nx * self.ny * self.nz)
