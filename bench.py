@@ -48,8 +48,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=64)
-    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample-steps", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB five times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE) and exit")
@@ -94,6 +95,43 @@ def cpu_baseline(case, steps):
     return {"value": rate, "unit": "grid-point RK-stage updates/s", "cores": threads, "kind": "port",
             "sample": f"{steps} RK3 step(s) of the full {case.cfg.nx}x{case.cfg.ny}x{case.cfg.nz} ramp, "
                       f"C oracle with {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def mixed_leg(case, solver64, args, value64):
+    """The same workload on the mixed-precision path (FP32 bulk; FP64 metrics,
+    WENO weights, Jacobian factors and RK state) from the same initial state
+    and for the same warmup + timed steps: its rate, its speedup over the
+    FP64 path and the relative L2 error of density and pressure against the
+    FP64 path's state at the same step."""
+    import numpy as np
+    from pyocfd import capi
+    cfg = capi.OcfdConfig.from_buffer_copy(case.cfg)
+    cfg.precision = capi.PREC_MIXED
+    m = case.load(capi.Solver(cfg, device=0))
+    if args.warmup:
+        m.step(args.warmup)
+    m.sync()
+    t0 = time.perf_counter()
+    m.step(args.steps)
+    m.sync()
+    wall = time.perf_counter() - t0
+    ms = m.last_step_ms()
+    secs = ms * 1e-3 if ms > 0 else wall
+    rate = case.npoints * 3 * args.steps / secs
+    (p64, i64, _), (p32, i32, _) = solver64.get_flow(), m.get_flow()
+    assert i64 == i32
+    g = case.cfg.gamma * case.cfg.ama ** 2
+    pr64, pr32 = p64[0] * p64[4] / g, p32[0] * p32[4] / g
+
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {"value": rate, "ms_per_step": secs * 1e3 / args.steps, "speedup": rate / value64,
+           "l2_rel_rho": l2(p32[0], p64[0]), "l2_rel_p": l2(pr32, pr64),
+           "max_rel_rho": float(np.abs(p32[0] - p64[0]).max() / np.abs(p64[0]).max()),
+           "after_steps": int(i64), "dtype": "f32 bulk / f64 metrics, weights, RK state"}
+    m.close()
+    return out
 
 
 def main():
@@ -141,6 +179,9 @@ def main():
     flags = solver.get_flags()
     import numpy as np
     cnt = np.bincount(np.concatenate([f.ravel() for f in flags]), minlength=4)[1:4]
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(case, solver, args, value)
     cpu = None
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         cpu = cpu_baseline(case, args.cpu_sample_steps)
@@ -165,6 +206,7 @@ def main():
         "wall_s": wall,
         "kernel_share": share,
         "roofline": roof,
+        "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

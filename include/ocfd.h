@@ -162,13 +162,63 @@ int ocfd_get_state(ocfd_ctx *ctx, double *f5);   /* conserved, [5][nz][ny][nx] *
 int ocfd_get_du(ocfd_ctx *ctx, double *du5);     /* last RHS, same layout */
 int ocfd_get_pressure(ocfd_ctx *ctx, double *p); /* interior p */
 int ocfd_get_amu(ocfd_ctx *ctx, double *amu);
-/* 13 metric arrays in OCFD3d-Jacobi.dat order, interior. */
+/* 13 metric arrays in OCFD3d-Jacobi.dat order, interior: the mesh
+ * coordinates as loaded, then the nine cofactors and Ajac. */
 int ocfd_get_metrics(ocfd_ctx *ctx, double *arrays13);
+/* Step diagnostics printed every KSTEP_SHOW steps by NS_solver_real
+ * (src/OCFD_NS_Solver.cu:124-152, ana_residual and ana_NAN_and_NT in
+ * src/OCFD_ana.cu:50-222): the means of the total energy f[4] and of T over
+ * the interior, and the numbers of NaN and of negative values of T over the
+ * field including its ghost layers.  Any pointer may be null. */
+int ocfd_diagnostics(ocfd_ctx *ctx, double *mean_energy, double *mean_T,
+                     long long *n_nan_T, long long *n_neg_T);
+/* Write OCFD3d-Jacobi.dat (the 13 arrays above) so that a later run reads the
+ * metrics instead of deriving them from OCFD3d-Mesh.dat.  The reference only
+ * reads this file (src/OCFD_Comput_Jacobian3d.cu:148-215) and has no writer. */
+int ocfd_write_jacobi(ocfd_ctx *ctx, const char *path);
 /* hybrid_auto flags: x (nx+1)*ny*nz, y nx*(ny+1)*nz, z nx*ny*(nz+1). */
 int ocfd_get_hybrid_flags(ocfd_ctx *ctx, int8_t *fx, int8_t *fy, int8_t *fz);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * context's stream and accumulated per kernel name. */
+/* ---- learned hybrid sensor (no counterpart in the reference) ----------
+ * A 24 -> 32 -> 32 -> 1 MLP (GELU, sigmoid) over 8 stencil features per
+ * direction (normalised p and rho jumps, the largest outer p jump, the
+ * contravariant velocity jump over c) gives the probability that a node needs
+ * WENO.  Parameters: 1889 floats, W1[32][24] b1[32] W2[32][32] b2[32] W3[32]
+ * b3.  With ocfd_set_sensor the hybrid_auto flags become: prob < lo -> UP7,
+ * prob > hi -> max(plain kp, WENO7), otherwise the plain style-2 kp.
+ * params = NULL switches back to the plain sensor. */
+int ocfd_set_sensor(ocfd_ctx *ctx, const float *params, float lo, float hi);
+/* Interior probabilities of the last learned-sensor pass (nz*ny*nx floats). */
+int ocfd_get_sensor_prob(ocfd_ctx *ctx, float *prob);
+
+/* Trainer: samples (24 features + plain label), parameters and Adam moments
+ * on one device.  Training is FP32 and deterministic for a given seed. */
+typedef struct ocfd_mlp ocfd_mlp;
+int ocfd_mlp_create(int device, long long capacity, ocfd_mlp **out);
+int ocfd_mlp_destroy(ocfd_mlp *m);
+int ocfd_mlp_set_params(ocfd_mlp *m, const float *params); /* resets Adam */
+int ocfd_mlp_get_params(ocfd_mlp *m, float *params);
+long long ocfd_mlp_sample_count(ocfd_mlp *m);
+int ocfd_mlp_clear(ocfd_mlp *m);
+/* Append the interior nodes whose indices are all multiples of stride, from
+ * the context's current state, with the plain sensor's label (kp > 1). */
+int ocfd_mlp_collect(ocfd_mlp *m, ocfd_ctx *ctx, int stride);
+int ocfd_mlp_put_samples(ocfd_mlp *m, const float *X, const float *y, long long n);
+int ocfd_mlp_get_samples(ocfd_mlp *m, long long first, long long n, float *X, float *y);
+/* Batch-mean weighted BCE and its gradient (1889 floats) on samples
+ * [first, first+n); positives weighted by wpos. */
+int ocfd_mlp_grad(ocfd_mlp *m, long long first, int n, float wpos, float *grads, float *loss);
+/* FP32 forward probabilities of samples [first, first+n). */
+int ocfd_mlp_predict(ocfd_mlp *m, long long first, int n, float *prob);
+/* Adam (lr; beta 0.9, 0.999) over shuffled mini-batches; epoch_loss (may be
+ * NULL) receives the mean loss of each epoch. */
+int ocfd_mlp_train(ocfd_mlp *m, int epochs, int batch, float lr, float wpos, unsigned long long seed,
+                   float *epoch_loss);
+/* ocfd_set_sensor with the trainer's current parameters. */
+int ocfd_set_sensor_from(ocfd_ctx *ctx, ocfd_mlp *m, float lo, float hi);
+
 int ocfd_profile_enable(ocfd_ctx *ctx, int on);
 int ocfd_profile_count(ocfd_ctx *ctx);
 /* i-th entry: name (NUL-terminated, <=63 chars), launches, total ms. */

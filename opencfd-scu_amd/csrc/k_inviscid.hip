@@ -253,6 +253,23 @@ __global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
     auto lin = [&](int s, int a) -> long long { return DIR == 1 ? gidx(g, a, s, k2) : gidx(g, a, k2, s); };
     auto at = [](int v, int n, int a) { return (v * NN + n) * NA + a; };
 
+    // the du and Ajac of this thread's output nodes are fetched first, so
+    // that their latency hides behind phases 1 and 2
+    constexpr int NPT = (NS * NA + 255) / 256;
+    double pre_du[NPT][5], pre_jac[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) {
+        const int idx = tid + 256 * t;
+        const int a = idx % NA, n = idx / NA;
+        const int i = s0 + n, ac = a0 + a;
+        if (idx < NS * NA && i < n_dir && ac < g.nx) {
+            const long long q = lin(i, ac);
+            pre_jac[t] = A.jac[q];
+#pragma unroll
+            for (int m = 0; m < 5; m++) pre_du[t][m] = double(A.du[(long long)m * A.fs + q]);
+        }
+    }
+
     // ---- phase 1: split of NN x NA nodes into LDS
     for (int idx = tid; idx < NN * NA; idx += 256) {
         const int a = idx % NA, n = idx / NA;
@@ -320,12 +337,14 @@ __global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
 
     // ---- phase 3: du at nodes (P then M; Jacobian factor and sum in double)
     const double rh = 1.0 / A.h;
-    for (int idx = tid; idx < NS * NA; idx += 256) {
+#pragma unroll
+    for (int t = 0; t < NPT; t++) {
+        const int idx = tid + 256 * t;
         const int a = idx % NA, n = idx / NA;
         const int i = s0 + n, ac = a0 + a;
-        if (i < n_dir && ac < g.nx) {
+        if (idx < NS * NA && i < n_dir && ac < g.nx) {
             const long long q = lin(i, ac);
-            const double ajac = A.jac[q];
+            const double ajac = pre_jac[t];
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
 #pragma unroll
@@ -334,11 +353,10 @@ __global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
                 const S mr = sh[((5 + m) * NF + n + 1) * NA + a], ml = sh[((5 + m) * NF + n) * NA + a];
                 const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
                 const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
-                S* dv = A.du + (long long)m * A.fs;
-                double acc = double(dv[q]);
+                double acc = pre_du[t][m];
                 acc = acc + tP;
                 acc = acc + tM;
-                dv[q] = S(acc);
+                A.du[(long long)m * A.fs + q] = S(acc);
             }
         }
     }

@@ -349,6 +349,69 @@ __global__ void k_set(double* p, long long n, double v) {
     if (t < n) p[t] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Step diagnostics of NS_solver_real (src/OCFD_NS_Solver.cu:124-152):
+// ana_residual sums of the total energy f[4] and of T over the interior
+// (src/OCFD_ana.cu:50-104), and the NaN / negative-T scan of
+// ana_NAN_and_NT over the LAP box (src/OCFD_ana.cu:105-222).  One partial
+// per block, reduced in a fixed order (LDS tree here, the block index order
+// on the host): the sums do not depend on scheduling.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256) k_diag_sums(Geo g, const double* __restrict__ E, const S* __restrict__ T,
+                                                   double* partE, double* partT) {
+    __shared__ double se[256], st[256];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    double e = 0.0, t = 0.0;
+    if (i < g.nx && j < g.ny) {
+        const long long q = gidx(g, i, j, k);
+        e = E[q];
+        t = double(T[q]);
+    }
+    se[tid] = e;
+    st[tid] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            se[tid] += se[tid + w];
+            st[tid] += st[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        partE[b] = se[0];
+        partT[b] = st[0];
+    }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_diag_scan(const S* __restrict__ T, long long n, int* part) {
+    __shared__ int sn[256], sg[256];
+    const int tid = threadIdx.x;
+    int nn = 0, ng = 0;
+    for (long long q = (long long)blockIdx.x * 256 + tid; q < n; q += (long long)gridDim.x * 256) {
+        const S v = T[q];
+        nn += (v != v);
+        ng += (v < S(0));
+    }
+    sn[tid] = nn;
+    sg[tid] = ng;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sn[tid] += sn[tid + w];
+            sg[tid] += sg[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[2 * blockIdx.x] = sn[0];
+        part[2 * blockIdx.x + 1] = sg[0];
+    }
+}
+
 template <typename To, typename From>
 __global__ void k_convert(To* dst, const From* src, long long n) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -444,6 +507,16 @@ void launch_convert(To* dst, const From* src, long long n, hipStream_t st) {
     hipLaunchKernelGGL((k_convert<To, From>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
 }
 
+int diag_sum_blocks(const Geo& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
+
+template <typename S>
+void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, double* partT, int* scan_part,
+                 hipStream_t st) {
+    hipLaunchKernelGGL(k_diag_sums<S>, pgrid(g), dim3(64, 4), 0, st, g, E, T_base, partE, partT);
+    // the LAP box is the first sz*(nz+8) elements of the field
+    hipLaunchKernelGGL(k_diag_scan<S>, dim3(DIAG_SCAN_BLOCKS), dim3(256), 0, st, T_base, g.sz * (g.nz + 8), scan_part);
+}
+
 #define OCFD_INSTANTIATE(S)                                                                                     \
     template void launch_fill<S>(const Geo&, S*, int, long long, int, hipStream_t);                          \
     template void launch_stage<S>(const StageArgsT<S>&, hipStream_t);                                        \
@@ -454,7 +527,8 @@ void launch_convert(To* dst, const From* src, long long n, hipStream_t st) {
     template void launch_filter<S>(const FilterArgsT<S>&, hipStream_t);                                      \
     template void launch_p2c_all<S>(const BCArgsT<S>&, hipStream_t);                                         \
     template void launch_c2p_all<S>(const StageArgsT<S>&, hipStream_t);                                      \
-    template void launch_amu_all<S>(const BCArgsT<S>&, hipStream_t);
+    template void launch_amu_all<S>(const BCArgsT<S>&, hipStream_t);                                        \
+    template void launch_diag<S>(const Geo&, const double*, const S*, double*, double*, int*, hipStream_t);
 OCFD_INSTANTIATE(double)
 OCFD_INSTANTIATE(float)
 #undef OCFD_INSTANTIATE

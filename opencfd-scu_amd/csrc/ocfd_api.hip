@@ -9,7 +9,9 @@
 //   refresh }; Istep++; tt+=dt; filter.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -64,9 +66,16 @@ struct ocfd_ctx {
     double* grad = nullptr;  // 12 fields of scratch (metric set-up, precision conversions)
     int8_t* flag[3] = {};
     double *ub1 = nullptr, *fxd = nullptr, *gzd = nullptr;
+    float* mlp_params = nullptr;  // learned sensor (null: the plain sensor)
+    float* mlp_store = nullptr;   // device copy of the parameters
+    float* mlp_prob = nullptr;    // its probability per node (LAP layout)
+    float mlp_lo = 0.2f, mlp_hi = 0.8f;
+    double* diag_part = nullptr;  // 2 * diag_sum_blocks partial sums
+    int* diag_scan = nullptr;     // 2 * DIAG_SCAN_BLOCKS counts
     // host state
     std::vector<double> coord_x, coord_z;  // mesh x, z (interior, for IBC 108 set-up)
     std::vector<double> coord_top_x;       // x along the top row at k=0
+    std::vector<double> mesh;              // x, y, z (interior, 3 arrays) as loaded
     std::vector<double> inlet;             // [4][ny]
     bool have_inlet = false, have_grid = false, have_flow = false, inited = false;
     double TM[64] = {}, fait[64] = {};
@@ -311,11 +320,33 @@ int apply_bc(ocfd_ctx* x) {
 }
 
 template <typename S>
+MlpSensorArgsT<S> mlp_args(ocfd_ctx* x) {
+    const View<S> V = view<S>(x);
+    MlpSensorArgsT<S> a;
+    a.g = x->g;
+    a.c = x->c;
+    a.d = V.d; a.u = V.u; a.v = V.v; a.w = V.w; a.T = V.T;
+    for (int m = 0; m < M_COUNT; m++) a.M[m] = x->M[m];
+    a.params = x->mlp_params;
+    a.lo = x->mlp_lo;
+    a.hi = x->mlp_hi;
+    a.fx = x->flag[0]; a.fy = x->flag[1]; a.fz = x->flag[2];
+    a.prob = x->mlp_prob;
+    return a;
+}
+
+template <typename S>
 int rhs(ocfd_ctx* x, int krk) {
     const ocfd_config& C = x->cfg;
     const View<S> V = view<S>(x);
-    if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1)
-        TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
+    if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1) {
+        if (x->mlp_params) {
+            const MlpSensorArgsT<S> ma = mlp_args<S>(x);
+            TRY(timed(x, "sensor_mlp", [&] { launch_mlp_sensor(ma, x->st); }));
+        } else {
+            TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
+        }
+    }
     TRY(fill_prims<S>(x));
     static const char* names[3] = {"sweep_x", "sweep_y", "sweep_z"};
     const int mk[3][3] = {{M_KX, M_KY, M_KZ}, {M_IX, M_IY, M_IZ}, {M_SX, M_SY, M_SZ}};
@@ -693,8 +724,13 @@ int ocfd_load_mesh(ocfd_ctx* x, const double* X, const double* Y, const double* 
     TRY(timed(x, "cofactors", [&] { launch_cofactors(g, x->grad, x->fs, x->M, x->st); }));
     TRY(fill_periodic(x, x->M[0], M_COUNT, "metric_fill"));
     HIPCK(hipStreamSynchronize(x->st));
-    // host copies for the IBC 108 set-up
+    // host copies for the IBC 108 set-up and the Jacobi writer
     const size_t nxy = (size_t)g.nx * g.ny;
+    const size_t nall = nxy * g.nz;
+    x->mesh.resize(3 * nall);
+    std::memcpy(x->mesh.data(), X, sizeof(double) * nall);
+    std::memcpy(x->mesh.data() + nall, Y, sizeof(double) * nall);
+    std::memcpy(x->mesh.data() + 2 * nall, Z, sizeof(double) * nall);
     x->coord_x.assign(X, X + g.nx);                         // x at (i, 0, 0)
     x->coord_top_x.assign(X + (size_t)(g.ny - 1) * g.nx, X + (size_t)(g.ny - 1) * g.nx + g.nx);
     x->coord_z.resize(g.nz);
@@ -714,6 +750,7 @@ int ocfd_load_jacobi(ocfd_ctx* x, const double* a13) {
     const double* X = a13;
     const double* Z = a13 + 2 * n;
     const size_t nxy = (size_t)g.nx * g.ny;
+    x->mesh.assign(a13, a13 + 3 * n);
     x->coord_x.assign(X, X + g.nx);
     x->coord_top_x.assign(X + (size_t)(g.ny - 1) * g.nx, X + (size_t)(g.ny - 1) * g.nx + g.nx);
     x->coord_z.resize(g.nz);
@@ -835,10 +872,89 @@ int ocfd_get_amu(ocfd_ctx* x, double* a) {
 int ocfd_get_metrics(ocfd_ctx* x, double* a13) {
     if (!x || !a13) return fail(OCFD_ERR_ARG, "null argument");
     const size_t n = (size_t)x->g.nx * x->g.ny * x->g.nz;
-    // coordinates are not kept on the device: return zeros for Axx Ayy Azz
-    std::memset(a13, 0, sizeof(double) * 3 * n);
+    if (x->mesh.size() == 3 * n) std::memcpy(a13, x->mesh.data(), sizeof(double) * 3 * n);
+    else std::memset(a13, 0, sizeof(double) * 3 * n);
     for (int m = 0; m < M_COUNT; m++) TRY(download_interior(x, x->M[m], a13 + (3 + m) * n));
     return 0;
+}
+
+int ocfd_diagnostics(ocfd_ctx* x, double* mean_energy, double* mean_T, long long* n_nan_T, long long* n_neg_T) {
+    if (!x || !x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
+    const Geo& g = x->g;
+    const int nb = diag_sum_blocks(g);
+    int rc = 0;
+    if (!x->diag_part) {
+        x->diag_part = (double*)alloc_bytes(x, sizeof(double) * 2 * nb, rc);
+        if (rc) return rc;
+        x->diag_scan = (int*)alloc_bytes(x, sizeof(int) * 2 * DIAG_SCAN_BLOCKS, rc);
+        if (rc) return rc;
+    }
+    const double* E = x->f + 4 * x->fs;
+    if (x->mixed)
+        TRY(timed(x, "diag", [&] { launch_diag(g, E, x->sT, x->diag_part, x->diag_part + nb, x->diag_scan, x->st); }));
+    else
+        TRY(timed(x, "diag", [&] { launch_diag(g, E, x->T, x->diag_part, x->diag_part + nb, x->diag_scan, x->st); }));
+    std::vector<double> part((size_t)2 * nb);
+    std::vector<int> cnt((size_t)2 * DIAG_SCAN_BLOCKS);
+    HIPCK(hipStreamSynchronize(x->st));
+    HIPCK(hipMemcpy(part.data(), x->diag_part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(cnt.data(), x->diag_scan, sizeof(int) * 2 * DIAG_SCAN_BLOCKS, hipMemcpyDeviceToHost));
+    double se = 0.0, st = 0.0;
+    for (int b = 0; b < nb; b++) { se += part[b]; st += part[nb + b]; }
+    long long nn = 0, ng = 0;
+    for (int b = 0; b < DIAG_SCAN_BLOCKS; b++) { nn += cnt[2 * b]; ng += cnt[2 * b + 1]; }
+    const double npts = (double)g.nx * g.ny * g.nz;  // ana_residual divides by the global point count
+    if (mean_energy) *mean_energy = se / npts;
+    if (mean_T) *mean_T = st / npts;
+    if (n_nan_T) *n_nan_T = nn;
+    if (n_neg_T) *n_neg_T = ng;
+    return 0;
+}
+
+int ocfd_set_sensor(ocfd_ctx* x, const float* params, float lo, float hi) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    if (!params) {
+        x->mlp_params = nullptr;  // the buffers stay allocated for a later call
+        return 0;
+    }
+    if (!(lo >= 0.f && hi <= 1.f && lo <= hi)) return fail(OCFD_ERR_ARG, "confidence band needs 0 <= lo <= hi <= 1");
+    int rc = 0;
+    if (!x->mlp_prob) {
+        float* pb = alloc_fields_f32(x, 1, rc);
+        if (rc) return rc;
+        float* pp = (float*)alloc_bytes(x, sizeof(float) * MLP_NPARAM, rc);
+        if (rc) return rc;
+        x->mlp_prob = pb;
+        x->mlp_store = pp;
+    }
+    HIPCK(hipMemcpyAsync(x->mlp_store, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, x->st));
+    HIPCK(hipStreamSynchronize(x->st));
+    x->mlp_params = x->mlp_store;
+    x->mlp_lo = lo;
+    x->mlp_hi = hi;
+    return 0;
+}
+
+int ocfd_get_sensor_prob(ocfd_ctx* x, float* prob) {
+    if (!x || !prob) return fail(OCFD_ERR_ARG, "null argument");
+    if (!x->mlp_prob) return fail(OCFD_ERR_STATE, "the learned sensor has not been set");
+    const Geo& g = x->g;
+    std::vector<float> h((size_t)x->fs);
+    HIPCK(hipStreamSynchronize(x->st));
+    HIPCK(hipMemcpy(h.data(), x->mlp_prob, sizeof(float) * x->fs, hipMemcpyDeviceToHost));
+    for (int k = 0; k < g.nz; k++)
+        for (int j = 0; j < g.ny; j++)
+            std::memcpy(prob + ((size_t)k * g.ny + j) * g.nx, &h[(size_t)gidx(g, 0, j, k)], sizeof(float) * g.nx);
+    return 0;
+}
+
+int ocfd_write_jacobi(ocfd_ctx* x, const char* path) {
+    if (!x || !path) return fail(OCFD_ERR_ARG, "null argument");
+    if (!x->have_grid) return fail(OCFD_ERR_STATE, "load the grid before writing OCFD3d-Jacobi.dat");
+    const Geo& g = x->g;
+    std::vector<double> a((size_t)13 * g.nx * g.ny * g.nz);
+    TRY(ocfd_get_metrics(x, a.data()));
+    return ocfd_write_records(path, g.nx, g.ny, g.nz, 13, a.data());
 }
 
 int ocfd_get_hybrid_flags(ocfd_ctx* x, int8_t* fx, int8_t* fy, int8_t* fz) {
@@ -858,6 +974,226 @@ int ocfd_get_hybrid_flags(ocfd_ctx* x, int8_t* fx, int8_t* fy, int8_t* fz) {
                         h[(size_t)gidx(g, i - (dd == 0), j - (dd == 1), k - (dd == 2))];
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Learned-sensor trainer: training samples, parameters and Adam moments on one
+// device.  No counterpart in the reference.
+// ---------------------------------------------------------------------------
+struct ocfd_mlp {
+    int device = 0;
+    hipStream_t st = nullptr;
+    long long cap = 0, n = 0;
+    float *X = nullptr, *y = nullptr;   // [cap][24], [cap]
+    float *P = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+    float *per = nullptr;               // per-sample vectors of one batch
+    int per_cap = 0;
+    int* idx = nullptr;                 // sample order of one epoch
+    float* loss = nullptr;              // one value per batch of an epoch
+    long long loss_cap = 0;
+    int adam_t = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+int mlp_alloc(ocfd_mlp* m, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return fail(OCFD_ERR_HIP, "hipMalloc failed (" + std::to_string(bytes) + " bytes)");
+    m->allocs.push_back(*p);
+    return 0;
+}
+int mlp_ensure_batch(ocfd_mlp* m, int nb) {
+    if (nb <= m->per_cap) return 0;
+    TRY(mlp_alloc(m, (void**)&m->per, sizeof(float) * mlp_per_sample_floats() * (size_t)nb));
+    m->per_cap = nb;
+    return 0;
+}
+// splitmix64: the seeded sample order of the training epochs
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+}  // namespace
+
+int ocfd_mlp_create(int device, long long capacity, ocfd_mlp** out) {
+    if (!out || capacity <= 0) return fail(OCFD_ERR_ARG, "bad argument");
+    *out = nullptr;
+    ocfd_mlp* m = new ocfd_mlp();
+    m->device = device;
+    m->cap = capacity;
+    int rc = 0;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess) {
+        delete m;
+        return fail(OCFD_ERR_HIP, "hipSetDevice / hipStreamCreate failed");
+    }
+    rc = mlp_alloc(m, (void**)&m->X, sizeof(float) * 24 * (size_t)capacity);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->y, sizeof(float) * (size_t)capacity);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->idx, sizeof(int) * (size_t)capacity);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->P, sizeof(float) * MLP_NPARAM);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->m1, sizeof(float) * MLP_NPARAM);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->m2, sizeof(float) * MLP_NPARAM);
+    if (!rc) rc = mlp_alloc(m, (void**)&m->grad, sizeof(float) * (MLP_NPARAM + 1));
+    if (rc) {
+        ocfd_mlp_destroy(m);
+        return rc;
+    }
+    // seeded He-style initial weights (uniform with the variance of the
+    // normal He initialisation), zero biases, zero Adam moments
+    std::vector<float> p(MLP_NPARAM, 0.f);
+    uint64_t s = 0x5EED5EEDULL;
+    auto uni = [&](float bound) { return (float)((splitmix(s) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * bound; };
+    for (int i = 0; i < 768; i++) p[i] = uni(std::sqrt(3.f * 2.f / 24.f));
+    for (int i = 800; i < 1824; i++) p[i] = uni(std::sqrt(3.f * 2.f / 32.f));
+    for (int i = 1856; i < 1888; i++) p[i] = uni(std::sqrt(3.f / 32.f));
+    if (hipMemcpy(m->P, p.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM) != hipSuccess ||
+        hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM) != hipSuccess) {
+        ocfd_mlp_destroy(m);
+        return fail(OCFD_ERR_HIP, "trainer initialisation failed");
+    }
+    *out = m;
+    return 0;
+}
+
+int ocfd_mlp_destroy(ocfd_mlp* m) {
+    if (!m) return 0;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    for (void* p : m->allocs) (void)hipFree(p);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+    return 0;
+}
+
+int ocfd_mlp_set_params(ocfd_mlp* m, const float* params) {
+    if (!m || !params) return fail(OCFD_ERR_ARG, "null argument");
+    HIPCK(hipMemcpy(m->P, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIPCK(hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM));
+    HIPCK(hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM));
+    m->adam_t = 0;
+    return 0;
+}
+
+int ocfd_mlp_get_params(ocfd_mlp* m, float* params) {
+    if (!m || !params) return fail(OCFD_ERR_ARG, "null argument");
+    HIPCK(hipStreamSynchronize(m->st));
+    HIPCK(hipMemcpy(params, m->P, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+long long ocfd_mlp_sample_count(ocfd_mlp* m) { return m ? m->n : 0; }
+
+int ocfd_mlp_clear(ocfd_mlp* m) {
+    if (!m) return fail(OCFD_ERR_ARG, "null argument");
+    m->n = 0;
+    return 0;
+}
+
+int ocfd_mlp_put_samples(ocfd_mlp* m, const float* X, const float* y, long long n) {
+    if (!m || !X || !y || n < 0) return fail(OCFD_ERR_ARG, "bad argument");
+    if (m->n + n > m->cap) return fail(OCFD_ERR_ARG, "trainer capacity exceeded");
+    HIPCK(hipMemcpy(m->X + 24 * m->n, X, sizeof(float) * 24 * (size_t)n, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(m->y + m->n, y, sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+    m->n += n;
+    return 0;
+}
+
+int ocfd_mlp_get_samples(ocfd_mlp* m, long long first, long long n, float* X, float* y) {
+    if (!m || first < 0 || n < 0 || first + n > m->n) return fail(OCFD_ERR_ARG, "bad range");
+    HIPCK(hipStreamSynchronize(m->st));
+    if (X) HIPCK(hipMemcpy(X, m->X + 24 * first, sizeof(float) * 24 * (size_t)n, hipMemcpyDeviceToHost));
+    if (y) HIPCK(hipMemcpy(y, m->y + first, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ocfd_mlp_collect(ocfd_mlp* m, ocfd_ctx* x, int stride) {
+    if (!m || !x || stride < 1) return fail(OCFD_ERR_ARG, "bad argument");
+    if (!x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
+    if (m->device != x->device) return fail(OCFD_ERR_ARG, "trainer and context are on different devices");
+    const long long n = mlp_collect_count(x->g, stride);
+    if (m->n + n > m->cap) return fail(OCFD_ERR_ARG, "trainer capacity exceeded");
+    float* X = m->X + 24 * m->n;
+    float* y = m->y + m->n;
+    if (x->mixed) {
+        const MlpSensorArgsT<float> a = mlp_args<float>(x);
+        TRY(timed(x, "mlp_collect", [&] { launch_mlp_collect(a, stride, X, y, x->st); }));
+    } else {
+        const MlpSensorArgsT<double> a = mlp_args<double>(x);
+        TRY(timed(x, "mlp_collect", [&] { launch_mlp_collect(a, stride, X, y, x->st); }));
+    }
+    HIPCK(hipStreamSynchronize(x->st));
+    m->n += n;
+    return 0;
+}
+
+int ocfd_mlp_grad(ocfd_mlp* m, long long first, int n, float wpos, float* grads, float* loss) {
+    if (!m || n <= 0 || first < 0 || first + n > m->n) return fail(OCFD_ERR_ARG, "bad range");
+    TRY(mlp_ensure_batch(m, n));
+    launch_mlp_step(m->P, m->X + 24 * first, m->y + first, nullptr, n, wpos, m->per, m->grad, m->m1, m->m2, 0, 0.f,
+                    m->grad + MLP_NPARAM, m->st);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(m->st));
+    if (grads) HIPCK(hipMemcpy(grads, m->grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
+    if (loss) HIPCK(hipMemcpy(loss, m->grad + MLP_NPARAM, sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ocfd_mlp_train(ocfd_mlp* m, int epochs, int batch, float lr, float wpos, unsigned long long seed,
+                   float* epoch_loss) {
+    if (!m || epochs <= 0 || batch <= 0) return fail(OCFD_ERR_ARG, "bad argument");
+    if (m->n < batch) return fail(OCFD_ERR_STATE, "fewer samples than one batch");
+    TRY(mlp_ensure_batch(m, batch));
+    const long long nbat = m->n / batch;  // the remainder of an epoch's order is left out
+    if (nbat > m->loss_cap) {
+        TRY(mlp_alloc(m, (void**)&m->loss, sizeof(float) * (size_t)nbat));
+        m->loss_cap = nbat;
+    }
+    std::vector<int> order((size_t)m->n);
+    std::vector<float> lh((size_t)nbat);
+    uint64_t s = seed;
+    for (int e = 0; e < epochs; e++) {
+        for (long long i = 0; i < m->n; i++) order[(size_t)i] = (int)i;
+        for (long long i = m->n - 1; i > 0; i--) std::swap(order[(size_t)i], order[(size_t)(splitmix(s) % (uint64_t)(i + 1))]);
+        HIPCK(hipMemcpyAsync(m->idx, order.data(), sizeof(int) * (size_t)m->n, hipMemcpyHostToDevice, m->st));
+        for (long long b = 0; b < nbat; b++) {
+            m->adam_t++;
+            launch_mlp_step(m->P, m->X, m->y, m->idx + b * batch, batch, wpos, m->per, nullptr, m->m1, m->m2,
+                            m->adam_t, lr, m->loss + b, m->st);
+        }
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(m->st));  // the order buffer is rewritten next epoch
+        if (epoch_loss) {
+            HIPCK(hipMemcpy(lh.data(), m->loss, sizeof(float) * (size_t)nbat, hipMemcpyDeviceToHost));
+            double t = 0.0;
+            for (long long b = 0; b < nbat; b++) t += lh[(size_t)b];
+            epoch_loss[e] = (float)(t / (double)nbat);
+        }
+    }
+    return 0;
+}
+
+int ocfd_mlp_predict(ocfd_mlp* m, long long first, int n, float* prob) {
+    // FP32 forward on stored samples (the training arithmetic), for checks
+    if (!m || !prob || n <= 0 || first < 0 || first + n > m->n) return fail(OCFD_ERR_ARG, "bad range");
+    TRY(mlp_ensure_batch(m, n));
+    launch_mlp_step(m->P, m->X + 24 * first, m->y + first, nullptr, n, 1.f, m->per, nullptr, m->m1, m->m2, 0, 0.f,
+                    m->grad + MLP_NPARAM, m->st);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(m->st));
+    // dz3 = (prob - y) with weight 1: recover prob from the per-sample vectors
+    const size_t ps = mlp_per_sample_floats();
+    std::vector<float> per(ps * (size_t)n), y((size_t)n);
+    HIPCK(hipMemcpy(per.data(), m->per, sizeof(float) * ps * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(y.data(), m->y + first, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) prob[i] = per[(size_t)i * ps + ps - 2] + y[(size_t)i];
+    return 0;
+}
+
+int ocfd_set_sensor_from(ocfd_ctx* x, ocfd_mlp* m, float lo, float hi) {
+    if (!x || !m) return fail(OCFD_ERR_ARG, "null argument");
+    std::vector<float> p(MLP_NPARAM);
+    TRY(ocfd_mlp_get_params(m, p.data()));
+    return ocfd_set_sensor(x, p.data(), lo, hi);
 }
 
 int ocfd_profile_enable(ocfd_ctx* x, int on) {
@@ -1004,8 +1340,25 @@ void NS_solver_real(void) {
         total += g_ctx->last_step_ms;
         window += g_ctx->last_step_ms;
         if (C.kstep_show > 0 && g_ctx->istep % C.kstep_show == 0) {
+            // src/OCFD_NS_Solver.cu:124-152: mean total energy (NaN => the
+            // NaN / negative-T check, which aborts) and mean T
+            double E0 = 0.0, T0 = 0.0;
+            long long nnan = 0, nneg = 0;
+            SHIM(ocfd_diagnostics(g_ctx, &E0, &T0, &nnan, &nneg));
+            if (std::isnan(E0)) {
+                if (nneg > 10) {
+                    std::printf("Negative T points %lld > %d\n", nneg, 10);
+                    std::exit(1);
+                }
+                if (nnan) {
+                    std::printf("NAN occured , program Abort\n");
+                    std::exit(1);
+                }
+            }
             std::printf("%lf of %lf ( %d ) , using %lf\n", g_ctx->tt, C.end_time, g_ctx->istep, total * 1e-3);
             std::printf("%d steps GPU time %lf\n", C.kstep_show, window * 1e-3);
+            std::printf("Averaged Total Energy is %lf\n", E0);
+            std::printf("Averaged Total T is %lf\n\n", T0);
             window = 0.0;
         }
         if (C.kstep_save > 0 && g_ctx->istep % C.kstep_save == 0) {

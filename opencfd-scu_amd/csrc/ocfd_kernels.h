@@ -108,6 +108,43 @@ template <typename S> void launch_filter(const FilterArgsT<S>& a, hipStream_t st
 void launch_metric_deriv(const Geo& g, const double* coord, int dir, int bound, double* out, hipStream_t st);
 void launch_cofactors(const Geo& g, const double* D9, long long fs, double* const* M, hipStream_t st);
 void launch_set(const Geo& g, double* base, long long n, double value, hipStream_t st);
+// Step diagnostics: per-block partial sums of E = f[4] and T over the interior
+// (diag_sum_blocks(g) partials each) and NaN / negative counts of T over the
+// LAP box (2*DIAG_SCAN_BLOCKS ints).  T_base is the T field (index 0 = first
+// LAP-box element).
+constexpr int DIAG_SCAN_BLOCKS = 1024;
+int diag_sum_blocks(const Geo& g);
+template <typename S>
+void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, double* partT, int* scan_part,
+                 hipStream_t st);
+
+// Learned sensor (k_mlp.hip): parameters W1 [32][24], b1, W2 [32][32], b2,
+// W3, b3 as 1889 floats.
+constexpr int MLP_NPARAM = 1889;
+template <typename S>
+struct MlpSensorArgsT {
+    Geo g;
+    Consts c;
+    const S *d, *u, *v, *w, *T;
+    const double *M[M_COUNT];
+    const float *params;
+    float lo, hi;                 // confidence band that falls back to the plain sensor
+    int8_t *fx, *fy, *fz;
+    float *prob;                  // probability per node (LAP layout) or null
+};
+template <typename S> void launch_mlp_sensor(const MlpSensorArgsT<S>& a, hipStream_t st);
+// Features and plain labels of the interior nodes (i, j, k) with i, j, k
+// multiples of stride, in lattice order: mlp_collect_count(g, stride) rows.
+template <typename S> void launch_mlp_collect(const MlpSensorArgsT<S>& a, int stride, float* X, float* y, hipStream_t st);
+long long mlp_collect_count(const Geo& g, int stride);
+// One training step on the nb samples idx[0..nb) of (X, y) (idx null: rows
+// 0..nb): per-sample vectors into per (nb * mlp_per_sample_floats()), the
+// batch-mean gradient into grad (may be null), the mean loss into *loss, and
+// with adam_t > 0 the Adam update of params (moments m1, m2).
+void launch_mlp_step(float* params, const float* X, const float* y, const int* idx, int nb, float wpos, float* per,
+                     float* grad, float* m1, float* m2, int adam_t, float lr, float* loss, hipStream_t st);
+size_t mlp_per_sample_floats();
+
 // dst[i] = (To)src[i] for n elements (precision conversion of whole fields).
 template <typename To, typename From>
 void launch_convert(To* dst, const From* src, long long n, hipStream_t st);

@@ -127,6 +127,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
             "(hipcc --offload-arch=gfx950); there is no CPU fallback")
     lib = C.CDLL(str(p))
     dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
     vp = C.c_void_p
     sig = {
@@ -154,6 +155,24 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "ocfd_get_pressure": (C.c_int, [vp, dp]),
         "ocfd_get_amu": (C.c_int, [vp, dp]),
         "ocfd_get_metrics": (C.c_int, [vp, dp]),
+        "ocfd_write_jacobi": (C.c_int, [vp, C.c_char_p]),
+        "ocfd_set_sensor": (C.c_int, [vp, C.c_void_p, C.c_float, C.c_float]),
+        "ocfd_get_sensor_prob": (C.c_int, [vp, fp]),
+        "ocfd_mlp_create": (C.c_int, [C.c_int, C.c_longlong, C.POINTER(C.c_void_p)]),
+        "ocfd_mlp_destroy": (C.c_int, [vp]),
+        "ocfd_mlp_set_params": (C.c_int, [vp, fp]),
+        "ocfd_mlp_get_params": (C.c_int, [vp, fp]),
+        "ocfd_mlp_sample_count": (C.c_longlong, [vp]),
+        "ocfd_mlp_clear": (C.c_int, [vp]),
+        "ocfd_mlp_collect": (C.c_int, [vp, vp, C.c_int]),
+        "ocfd_mlp_put_samples": (C.c_int, [vp, fp, fp, C.c_longlong]),
+        "ocfd_mlp_get_samples": (C.c_int, [vp, C.c_longlong, C.c_longlong, fp, fp]),
+        "ocfd_mlp_grad": (C.c_int, [vp, C.c_longlong, C.c_int, C.c_float, fp, C.POINTER(C.c_float)]),
+        "ocfd_mlp_predict": (C.c_int, [vp, C.c_longlong, C.c_int, fp]),
+        "ocfd_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_ulonglong, fp]),
+        "ocfd_set_sensor_from": (C.c_int, [vp, vp, C.c_float, C.c_float]),
+        "ocfd_diagnostics": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ocfd_get_hybrid_flags": (C.c_int, [vp, i8p, i8p, i8p]),
         "ocfd_profile_enable": (C.c_int, [vp, C.c_int]),
         "ocfd_profile_count": (C.c_int, [vp]),
@@ -208,6 +227,88 @@ def read_records(path, nx, ny, nz, nfield):
     buf = np.empty(nfield * nx * ny * nz)
     _check(load_library().ocfd_read_records(str(path).encode(), nx, ny, nz, nfield, buf), "read_records")
     return buf.reshape(nfield, nz, ny, nx)
+
+
+MLP_NPARAM = 1889
+
+
+def mlp_pack(p) -> np.ndarray:
+    """Dict W1 b1 W2 b2 W3 b3 -> the 1889-float parameter vector."""
+    return np.concatenate([np.ravel(p[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3")]).astype(np.float32)
+
+
+def mlp_unpack(v) -> dict:
+    v = np.asarray(v)
+    return {"W1": v[0:768].reshape(32, 24), "b1": v[768:800], "W2": v[800:1824].reshape(32, 32),
+            "b2": v[1824:1856], "W3": v[1856:1888], "b3": v[1888:1889]}
+
+
+class Trainer:
+    """The learned-sensor trainer (ocfd_mlp) on one device."""
+
+    def __init__(self, capacity, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.ocfd_mlp_create(device, int(capacity), C.byref(h)), "mlp_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ocfd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, v):
+        _check(self.lib.ocfd_mlp_set_params(self.h, np.ascontiguousarray(v, np.float32)), "mlp_set_params")
+
+    def get_params(self):
+        out = np.empty(MLP_NPARAM, np.float32)
+        _check(self.lib.ocfd_mlp_get_params(self.h, out), "mlp_get_params")
+        return out
+
+    @property
+    def n(self):
+        return self.lib.ocfd_mlp_sample_count(self.h)
+
+    def clear(self):
+        _check(self.lib.ocfd_mlp_clear(self.h), "mlp_clear")
+
+    def collect(self, solver, stride):
+        _check(self.lib.ocfd_mlp_collect(self.h, solver.h, int(stride)), "mlp_collect")
+
+    def put(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        _check(self.lib.ocfd_mlp_put_samples(self.h, X, y, len(y)), "mlp_put_samples")
+
+    def get(self, first=0, n=None):
+        n = self.n - first if n is None else n
+        X = np.empty((n, 24), np.float32)
+        y = np.empty(n, np.float32)
+        _check(self.lib.ocfd_mlp_get_samples(self.h, first, n, X, y), "mlp_get_samples")
+        return X, y
+
+    def grad(self, first, n, wpos=1.0):
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_float()
+        _check(self.lib.ocfd_mlp_grad(self.h, first, n, wpos, g, C.byref(loss)), "mlp_grad")
+        return loss.value, g
+
+    def predict(self, first=0, n=None):
+        n = self.n - first if n is None else n
+        out = np.empty(n, np.float32)
+        _check(self.lib.ocfd_mlp_predict(self.h, first, n, out), "mlp_predict")
+        return out
+
+    def train(self, epochs, batch, lr, wpos=1.0, seed=1):
+        out = np.empty(epochs, np.float32)
+        _check(self.lib.ocfd_mlp_train(self.h, epochs, batch, lr, wpos, seed, out), "mlp_train")
+        return out
 
 
 class Solver:
@@ -295,6 +396,30 @@ class Solver:
         out = np.empty(13 * self.nx * self.ny * self.nz)
         _check(self.lib.ocfd_get_metrics(self.h, out), "get_metrics")
         return out.reshape((13,) + self.shape)
+
+    def diagnostics(self):
+        """(mean total energy, mean T, NaN count of T, negative-T count)."""
+        e, t, nn, ng = C.c_double(), C.c_double(), C.c_longlong(), C.c_longlong()
+        _check(self.lib.ocfd_diagnostics(self.h, C.byref(e), C.byref(t), C.byref(nn), C.byref(ng)), "diagnostics")
+        return e.value, t.value, nn.value, ng.value
+
+    def set_sensor(self, params=None, lo=0.2, hi=0.8):
+        """Learned hybrid sensor with these 1889 parameters (None: plain)."""
+        if params is None:
+            _check(self.lib.ocfd_set_sensor(self.h, None, 0.0, 1.0), "set_sensor")
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAM
+        _check(self.lib.ocfd_set_sensor(self.h, p.ctypes.data_as(C.c_void_p), lo, hi), "set_sensor")
+
+    def get_sensor_prob(self):
+        out = np.empty(self.nx * self.ny * self.nz, np.float32)
+        _check(self.lib.ocfd_get_sensor_prob(self.h, out), "get_sensor_prob")
+        return out.reshape(self.shape)
+
+    def write_jacobi(self, path):
+        """OCFD3d-Jacobi.dat with the coordinates and metrics of this context."""
+        _check(self.lib.ocfd_write_jacobi(self.h, str(path).encode()), "write_jacobi")
 
     def get_flags(self):
         nx, ny, nz = self.nx, self.ny, self.nz

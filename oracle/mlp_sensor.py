@@ -21,6 +21,9 @@ direction (A = the direction's cofactor row) at the nodes n-3..n+4 of the
   6     max over the outer intervals m = -3, -2, 2, 3 of |p jump|
   7     (U[n+1] - U[n]) / c[n],  c = sqrt(T)/Ma
 
+A jump whose denominator is 0 (ghost layers that are never written hold 0)
+is 0, and so is feature 7 where c = 0.
+
 Network: 24 -> 32 -> 32 -> 1, GELU (tanh form) on the hidden layers,
 sigmoid output = probability that the point needs WENO (plain kp > 1).
 Flag rule: prob < lo -> 1 (UP7); prob > hi -> max(kp, 2); otherwise the
@@ -105,7 +108,7 @@ def jameson_kp(P, i1, i2):
     nx+2) -> (nz, ny, nx) int8."""
     c = P[1:-1, 1:-1, 1:-1]
     dp = 0.0
-    for ax in range(3):
+    for ax in (2, 1, 0):  # x, y, z: the summation order of the sensor
         sl_m = [slice(1, -1)] * 3
         sl_p = [slice(1, -1)] * 3
         sl_m[ax] = slice(0, -2)
@@ -121,12 +124,40 @@ def features_line(p, rho, U, c, n):
     q = n + 4
 
     def jump(a, m):
-        return (a[q + m + 1] - a[q + m]) / (a[q + m + 1] + a[q + m])
+        s = a[q + m + 1] + a[q + m]
+        return (a[q + m + 1] - a[q + m]) / s if s != 0 else 0.0
 
     f = [jump(p, -1), jump(p, 0), jump(p, 1), jump(rho, -1), jump(rho, 0), jump(rho, 1)]
     f.append(max(abs(jump(p, m)) for m in (-3, -2, 2, 3)))
-    f.append((U[q + 1] - U[q]) / c[q])
+    f.append((U[q + 1] - U[q]) / c[q] if c[q] != 0 else 0.0)
     return np.array(f)
+
+
+def features_field(d, u, v, w, T, A, p00, rama, nodes):
+    """Features (len(nodes), 24) from LAP-box fields (nz+8, ny+8, nx+8) of the
+    primitives and the metric cofactors A (9, ...) in the order Akx Aky Akz
+    Aix Aiy Aiz Asx Asy Asz; nodes = (i, j, k) interior indices (>= -1)."""
+    P = p00 * d * T
+    c = np.sqrt(np.maximum(T, 0.0)) * rama
+    out = np.zeros((len(nodes), 24))
+    for r, (i, j, k) in enumerate(nodes):
+        for dd in range(3):
+            a1, a2, a3 = A[3 * dd], A[3 * dd + 1], A[3 * dd + 2]
+            idx = []
+            for o in range(-4, 5):
+                ii, jj, kk = i + 4 + (o if dd == 0 else 0), j + 4 + (o if dd == 1 else 0), k + 4 + (o if dd == 2 else 0)
+                idx.append((kk, jj, ii))
+            pl = np.array([P[t] for t in idx] + [0.0] * 3)
+            rl = np.array([d[t] for t in idx] + [0.0] * 3)
+            cl = np.array([c[t] for t in idx] + [0.0] * 3)
+            Ul = np.zeros(12)
+            for m in (4, 5):
+                t = idx[m]
+                nrm = np.sqrt(a1[t] ** 2 + a2[t] ** 2 + a3[t] ** 2)
+                Ul[m] = (a1[t] * u[t] + a2[t] * v[t] + a3[t] * w[t]) / nrm
+            # line arrays indexed by node + 4 with node 0 = this node
+            out[r, 8 * dd:8 * dd + 8] = features_line(pl, rl, Ul, cl, 0)
+    return out
 
 
 def flag_rule(prob, kp, lo=0.2, hi=0.8):
