@@ -135,6 +135,119 @@ __device__ __forceinline__ void face_pm(const R* gp, const R* gm, int c, int r, 
     }
 }
 
+// The same selection for all five variables of a face: the closure/scheme
+// case of the P and the M side is decided once per face, then each case loops
+// over the variables (ld(v, k) = f+ of variable v (v < 5) or f- of variable
+// v-5 at stencil node k, k = 0..7 for nodes j-3..j+4).  Same arithmetic as
+// face_pm, with the per-variable branching hoisted.
+template <typename R, typename W, int SCH, typename LD>
+__device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, int hy, R tvlim, R tvmax, R* hp,
+                                         R* hm) {
+    constexpr int kaP = Ext<SCH, 0>::ka, kbP = Ext<SCH, 0>::kb;
+    constexpr int kaM = Ext<SCH, 1>::ka, kbM = Ext<SCH, 1>::kb;
+    // P: 0..3 left closures (c), 4..6 right closures, 7 scheme, 8 WENO7, 9 WENO5
+    int pc;
+    if (bl && c <= -kaP) pc = c;
+    else if (br && r >= -kbP) pc = r == -1 ? 4 : r == -2 ? 5 : 6;
+    else if (SCH == OCFD_HYBRIDAUTO) pc = hy == 1 ? 7 : hy == 2 ? 8 : 9;
+    else pc = 7;
+    if (pc == 7) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) g[k] = ld(m, k);
+            if (SCH == OCFD_WENO7) hp[m] = weno7_up<R, W>(g);
+            else if (SCH == OCFD_UP7 || SCH == OCFD_HYBRIDAUTO) hp[m] = up7_up(g);
+            else if (SCH == OCFD_WENO5) hp[m] = weno5_up<R, W>(g + 1);
+            else if (SCH == OCFD_NND2) hp[m] = nnd2_up(g + 2);
+            else if (SCH == OCFD_WENO7_SYMBO) hp[m] = symbo_up<R, W>(g, 0, tvlim, tvmax);
+            else if (SCH == OCFD_WENO7_SYMBO_LIM) hp[m] = symbo_up<R, W>(g, 1, tvlim, tvmax);
+            else if (SCH == OCFD_OMP6_HR) hp[m] = omp6_up(0, g);
+            else if (SCH == OCFD_OMP6_LD) hp[m] = omp6_up(1, g);
+            else hp[m] = omp6_up(2, g);
+        }
+    } else if (pc == 8) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 0; k < 7; k++) g[k] = ld(m, k);
+            hp[m] = weno7_up<R, W>(g);
+        }
+    } else if (pc == 9 || pc == 3 || pc == 6) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 1; k < 6; k++) g[k] = ld(m, k);
+            hp[m] = weno5_up<R, W>(g + 1);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) g[k] = ld(m, k);
+            if (pc == 0) hp[m] = g[4];
+            else if (pc == 1) hp[m] = weno5_up_left<R, W>(g + 1);
+            else if (pc == 2) hp[m] = weno5_up_leftplus<R, W>(g + 1);
+            else if (pc == 4) hp[m] = g[3] + R(0.5) * d_minmod2(g[3] - g[2], g[3] - g[2]);
+            else hp[m] = weno5_up_rightplus<R, W>(g + 1);
+        }
+    }
+    // M: 0..2 left closures (c), 3..6 right closures, 7 scheme, 8 WENO7, 9 WENO5
+    int mc;
+    if (bl && c < -kaM) mc = c;
+    else if (br && r >= -kbM - 1) mc = r == -1 ? 3 : r == -2 ? 4 : r == -3 ? 5 : 6;
+    else if (SCH == OCFD_HYBRIDAUTO) mc = hy == 1 ? 7 : hy == 2 ? 8 : 9;
+    else mc = 7;
+    if (mc == 7) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) g[k] = ld(5 + m, k);
+            if (SCH == OCFD_WENO7) hm[m] = weno7_dn<R, W>(g + 1);
+            else if (SCH == OCFD_UP7 || SCH == OCFD_HYBRIDAUTO) hm[m] = up7_dn(g);
+            else if (SCH == OCFD_WENO5) hm[m] = weno5_dn<R, W>(g + 2);
+            else if (SCH == OCFD_NND2) hm[m] = nnd2_dn(g + 3);
+            else if (SCH == OCFD_WENO7_SYMBO || SCH == OCFD_WENO7_SYMBO_LIM) hm[m] = symbo_dn<R, W>(g);
+            else if (SCH == OCFD_OMP6_HR) hm[m] = omp6_dn(0, g);
+            else if (SCH == OCFD_OMP6_LD) hm[m] = omp6_dn(1, g);
+            else hm[m] = omp6_dn(2, g);
+        }
+    } else if (mc == 8) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 1; k < 8; k++) g[k] = ld(5 + m, k);
+            hm[m] = weno7_dn<R, W>(g + 1);
+        }
+    } else if (mc == 9 || mc == 2 || mc == 6) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 2; k < 7; k++) g[k] = ld(5 + m, k);
+            hm[m] = weno5_dn<R, W>(g + 2);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            R g[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) g[k] = ld(5 + m, k);
+            if (mc == 0) hm[m] = weno5_dn_left<R, W>(g + 2);
+            else if (mc == 1) hm[m] = weno5_dn_leftplus<R, W>(g + 2);
+            else if (mc == 3) hm[m] = g[3];
+            else if (mc == 4) hm[m] = g[4] - R(0.5) * d_minmod2(g[4] - g[3], g[4] - g[3]);
+            else hm[m] = weno5_dn_rightplus<R, W>(g + 2);
+        }
+    }
+}
+
 // x sweep: a block owns 4 rows of 56 output nodes; the f+/f- of the 64 nodes
 // of a row (4 ghost nodes each side) are staged in LDS.
 constexpr int XS = 56, XA = 4, XN = XS + 8, XF = XS + 1;
@@ -145,7 +258,8 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
     __shared__ S sh[10 * XN * XA];
     const Geo& g = A.g;
     const int tid = threadIdx.x;
-    const int s0 = blockIdx.x * XS, a0 = blockIdx.y * XA, k2 = blockIdx.z;
+    const Blk B = xcd_block();
+    const int s0 = B.x * XS, a0 = B.y * XA, k2 = B.z;
     const int n_dir = g.nx;
     // LDS element (v, n, a): node-contiguous so consecutive lanes hold
     // consecutive x
@@ -186,16 +300,8 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
             int hy = 0;
             if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[gidx(g, j, ac, k2)];
             const int c = j + 1, r = j - n_dir;
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                S gp[8], gm[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    gp[k] = sh[at(m, f_loc + k, a_loc)];
-                    gm[k] = sh[at(5 + m, f_loc + k, a_loc)];
-                }
-                face_pm<S, double, SCH>(gp, gm, c, r, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max), hp[m], hm[m]);
-            }
+            face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
+                                     S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
         }
     }
     __syncthreads();
@@ -239,8 +345,13 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
 // NA*8-byte row segment per node) and NS output nodes along the sweep, for one
 // value of the third index.  Same three phases as the x sweep; a thread may
 // handle up to two faces and two nodes.
+#ifdef OCFD_TILE_WPE
+#define TILE_ATTR __attribute__((amdgpu_waves_per_eu(OCFD_TILE_WPE, 8)))
+#else
+#define TILE_ATTR
+#endif
 template <int DIR, int SCH, int NS, int NA, typename S>
-__global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
+__global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
     static_assert(DIR == 1 || DIR == 2, "x is swept by k_sweep");
     constexpr int NN = NS + 8, NF = NS + 1;
     constexpr int FPT = (NF * NA + 255) / 256;
@@ -248,7 +359,8 @@ __global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
     __shared__ S sh[10 * NN * NA];
     const Geo& g = A.g;
     const int tid = threadIdx.x;
-    const int a0 = blockIdx.x * NA, s0 = blockIdx.y * NS, k2 = blockIdx.z;
+    const Blk B = xcd_block();
+    const int a0 = B.x * NA, s0 = B.y * NS, k2 = B.z;
     const int n_dir = DIR == 1 ? g.ny : g.nz;
     auto lin = [&](int s, int a) -> long long { return DIR == 1 ? gidx(g, a, s, k2) : gidx(g, a, k2, s); };
     auto at = [](int v, int n, int a) { return (v * NN + n) * NA + a; };
@@ -305,17 +417,8 @@ __global__ void __launch_bounds__(256) k_tile(SweepArgsT<S> A) {
             int hy = 0;
             if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[lin(j, ac)];
             const int c = j + 1, r = j - n_dir;
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                S gp[8], gm[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    gp[k] = sh[at(m, f_loc + k, a_loc)];
-                    gm[k] = sh[at(5 + m, f_loc + k, a_loc)];
-                }
-                face_pm<S, double, SCH>(gp, gm, c, r, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max), hp[t][m],
-                                        hm[t][m]);
-            }
+            face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
+                                     S(A.c.tv_lim), S(A.c.tv_max), hp[t], hm[t]);
         } else {
 #pragma unroll
             for (int m = 0; m < 5; m++) { hp[t][m] = S(0); hm[t][m] = S(0); }

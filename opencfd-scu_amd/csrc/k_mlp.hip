@@ -182,7 +182,8 @@ __global__ void __launch_bounds__(256) k_mlp_sensor(FeatIn<S> F, const float* __
         sb2[tid] = P[OB2 + tid];
         sW3[tid] = P[OW3 + tid];
     }
-    const int i = blockIdx.x * 64 + lane - 1, j = blockIdx.y * 4 + wv - 1, k = blockIdx.z - 1;
+    const Blk B = xcd_block();
+    const int i = B.x * 64 + lane - 1, j = B.y * 4 + wv - 1, k = B.z - 1;
     const bool own = i < g.nx && j < g.ny;
     const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
     float f[NF];

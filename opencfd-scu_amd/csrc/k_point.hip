@@ -44,74 +44,68 @@ template <typename S>
 __global__ void __launch_bounds__(256) k_stage(StageArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
-    const long long q = gidx(A.g, i, j, k), fs = A.fs;
-    const double dt = A.dt;
-    double fv[5];
+    const long long q = gidx(A.g, i, j, k);
+    double du[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
-        const long long o = m * fs + q;
-        const double du = double(A.du[o]);
-        double r;
-        if (A.krk == 1) {
-            r = A.fn[o] + dt * du;
-        } else if (A.krk == 2) {
-            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du);
-        } else {
-            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du);
-        }
-        A.f[o] = r;
-        if (A.flap) A.flap[o] = r;
-        fv[m] = r;
-    }
-    const double d1 = fv[0];
-    const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
-    const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
-    A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
-    A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
-    A.amu[q] = S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2));
+    for (int m = 0; m < 5; m++) du[m] = double(A.du[m * A.fs + q]);
+    stage_point(A, q, du);
 }
 
 // ---------------------------------------------------------------------------
-// IBC 108, bc_user_Compression_conner (src/OCFD_boundary_compression_conner.cu
-// 27-179), launched in the reference's order: inlet, top, wall velocity, wall
-// temperature/density with mirrored ghosts.  Interior indices; i<0 is ghost.
+// prim->cons of one point (pri_to_cons_kernel, src/commen_kernel.cu:32-64),
+// used by the face refresh of OCFD_bc (src/OCFD_boundary.cu:40-91) and at
+// initialisation.
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ void k_bc_inlet(BCArgsT<S> A) {  // do_ub1_inlet_kernel
-    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    if (j >= A.g.ny) return;
+__device__ __forceinline__ void p2c(const BCArgsT<S>& A, long long q) {
+    const double d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
+    A.f[q] = d;
+    A.f[A.fs + q] = u * d;
+    A.f[2 * A.fs + q] = v * d;
+    A.f[3 * A.fs + q] = w * d;
+    A.f[4 * A.fs + q] = d * ((u * u + v * v + w * w) * 0.5 + A.c.cv * T);
+}
+// ---------------------------------------------------------------------------
+// IBC 108, bc_user_Compression_conner (src/OCFD_boundary_compression_conner.cu
+// 27-179: inlet, top, wall velocity, wall temperature/density with mirrored
+// ghosts), get_Amu on the faces it changes, and the face refresh, in three
+// launches instead of the reference's eight, with the reference's order kept
+// where points are shared: (A) inlet and top (they
+// write the same values where they overlap), (B) per wall column the wall
+// velocity then the wall temperature/density and mirrored ghosts (which read
+// the inlet column at i = 0, so B follows A), (C) per face point the viscosity
+// (faces x=0, y=0, y=ny-1, when IBC 108 ran) and prim->cons on all six faces.
+template <typename S>
+__global__ void k_bc_inlet_top(BCArgsT<S> A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     const int ny = A.g.ny;
-    for (int i = -4; i <= 0; i++) {
-        const long long q = gidx(A.g, i, j, k);
-        A.d[q] = S(A.ub1[j]); A.u[q] = S(A.ub1[j + ny]); A.v[q] = S(A.ub1[j + 2 * ny]); A.w[q] = S(0);
-        A.T[q] = S(A.ub1[j + 3 * ny]);
+    if (t < ny) {
+        const int j = t;
+        for (int i = -4; i <= 0; i++) {
+            const long long q = gidx(A.g, i, j, k);
+            A.d[q] = S(A.ub1[j]); A.u[q] = S(A.ub1[j + ny]); A.v[q] = S(A.ub1[j + 2 * ny]); A.w[q] = S(0);
+            A.T[q] = S(A.ub1[j + 3 * ny]);
+        }
+    } else {
+        const int i = A.x_begin + (t - ny);
+        if (i >= A.g.nx) return;
+        const long long q = gidx(A.g, i, ny - 1, k);
+        A.d[q] = S(A.ub1[ny - 1]); A.u[q] = S(A.ub1[ny + ny - 1]); A.v[q] = S(A.ub1[2 * ny + ny - 1]); A.w[q] = S(0);
+        A.T[q] = S(A.ub1[3 * ny + ny - 1]);
     }
 }
 template <typename S>
-__global__ void k_bc_top(BCArgsT<S> A) {  // do_ub1_top_kernel
-    const int i = A.x_begin + blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    if (i >= A.g.nx) return;
-    const int ny = A.g.ny;
-    const long long q = gidx(A.g, i, ny - 1, k);
-    A.d[q] = S(A.ub1[ny - 1]); A.u[q] = S(A.ub1[ny + ny - 1]); A.v[q] = S(A.ub1[2 * ny + ny - 1]); A.w[q] = S(0);
-    A.T[q] = S(A.ub1[3 * ny + ny - 1]);
-}
-template <typename S>
-__global__ void k_bc_wall_vel(BCArgsT<S> A) {  // do_wall_dist_kernel
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    if (i >= A.g.nx) return;
-    const long long q = gidx(A.g, i, 0, k);
-    A.u[q] = S(0);
-    A.v[q] = S(A.epsl * A.fx[i] * A.gz[k] * A.ht);
-    A.w[q] = S(0);
-}
-template <typename S>
-__global__ void k_bc_wall_T(BCArgsT<S> A) {  // do_wall_Tp1_kernel / do_wall_Tp2_kernel
+__global__ void k_bc_wall(BCArgsT<S> A) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (i >= A.g.nx) return;
     const Geo& g = A.g;
+    {  // do_wall_dist_kernel
+        const long long q = gidx(g, i, 0, k);
+        A.u[q] = S(0);
+        A.v[q] = S(A.epsl * A.fx[i] * A.gz[k] * A.ht);
+        A.w[q] = S(0);
+    }
+    if (A.wall_not_normal != 0) return;
     const long long q0 = gidx(g, i, 0, k), q1 = gidx(g, i, 1, k), q2 = gidx(g, i, 2, k), q3 = gidx(g, i, 3, k);
     auto D = [&](long long q) { return double(A.d[q]); };
     auto T = [&](long long q) { return double(A.T[q]); };
@@ -133,46 +127,35 @@ __global__ void k_bc_wall_T(BCArgsT<S> A) {  // do_wall_Tp1_kernel / do_wall_Tp2
         A.d[q0] = S(pw / T(q0));
     }
 }
-// Viscosity on the faces IBC 108 may change (get_Amu runs after OCFD_bc,
-// src/OCFD_NS_Solver.cu:83-87): x=0, y=0, y=ny-1.
 template <typename S>
-__global__ void k_bc_amu(BCArgsT<S> A) {
+__global__ void k_faces(BCArgsT<S> A) {
     const Geo& g = A.g;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    int i, j;
-    if (t < g.ny) { i = 0; j = t; }
-    else if (t < g.ny + g.nx) { i = t - g.ny; j = 0; }
-    else if (t < g.ny + 2 * g.nx) { i = t - g.ny - g.nx; j = g.ny - 1; }
-    else return;
+    const long long nyz = (long long)g.ny * g.nz, nxz = (long long)g.nx * g.nz, nxy = (long long)g.nx * g.ny;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    bool mu = false;
+    if (t < 2 * nyz) {
+        const int side = (int)(t / nyz);
+        const long long r = t % nyz;
+        i = side ? g.nx - 1 : 0; j = (int)(r % g.ny); k = (int)(r / g.ny);
+        mu = side == 0;
+    } else if ((t -= 2 * nyz) < 2 * nxz) {
+        const int side = (int)(t / nxz);
+        const long long r = t % nxz;
+        i = (int)(r % g.nx); j = side ? g.ny - 1 : 0; k = (int)(r / g.nx);
+        mu = true;
+    } else if ((t -= 2 * nxz) < 2 * nxy) {
+        const int side = (int)(t / nxy);
+        const long long r = t % nxy;
+        i = (int)(r % g.nx); j = (int)(r / g.nx); k = side ? g.nz - 1 : 0;
+    } else {
+        return;
+    }
     const long long q = gidx(g, i, j, k);
-    const double T = double(A.T[q]);
-    A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
-}
-
-// Face refresh of OCFD_bc (src/OCFD_boundary.cu:40-91): prim->cons
-// (pri_to_cons_kernel, src/commen_kernel.cu:32-64) on all six faces.
-template <typename S>
-__device__ __forceinline__ void p2c(const BCArgsT<S>& A, long long q) {
-    const double d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
-    A.f[q] = d;
-    A.f[A.fs + q] = u * d;
-    A.f[2 * A.fs + q] = v * d;
-    A.f[3 * A.fs + q] = w * d;
-    A.f[4 * A.fs + q] = d * ((u * u + v * v + w * w) * 0.5 + A.c.cv * T);
-}
-template <typename S>
-__global__ void k_face_refresh(BCArgsT<S> A, int dir) {
-    const Geo& g = A.g;
-    const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * na * nb) return;
-    const int side = t / (na * nb), r = t % (na * nb), a = r % na, b = r / na;
-    const int n = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
-    const int s = side ? n - 1 : 0;
-    long long q;
-    if (dir == 0) q = gidx(g, s, a, b);
-    else if (dir == 1) q = gidx(g, a, s, b);
-    else q = gidx(g, a, b, s);
+    if (mu && A.amu) {
+        const double T = double(A.T[q]);
+        A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
+    }
     p2c(A, q);
 }
 
@@ -186,7 +169,8 @@ __global__ void k_face_refresh(BCArgsT<S> A, int dir) {
 template <typename S>
 __global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const S* __restrict__ d,
                                                 const S* __restrict__ T, int8_t* fx, int8_t* fy, int8_t* fz) {
-    const int i = blockIdx.x * 64 + threadIdx.x - 1, j = blockIdx.y * 4 + threadIdx.y - 1, k = blockIdx.z - 1;
+    const Blk B = xcd_block();
+    const int i = B.x * 64 + threadIdx.x - 1, j = B.y * 4 + threadIdx.y - 1, k = B.z - 1;
     if (i >= g.nx || j >= g.ny) return;
     const long long q = gidx(g, i, j, k);
     auto P = [&](long long o) { return c.p00 * double(d[o]) * double(T[o]); };
@@ -222,8 +206,9 @@ __device__ __forceinline__ double filt_sigma(const S* __restrict__ P, long long 
 }
 template <typename S>
 __global__ void __launch_bounds__(256) k_filter(FilterArgsT<S> A) {
-    const int i = A.lo[0] + blockIdx.x * 64 + threadIdx.x, j = A.lo[1] + blockIdx.y * 4 + threadIdx.y;
-    const int k = A.lo[2] + blockIdx.z;
+    const Blk B = xcd_block();
+    const int i = A.lo[0] + B.x * 64 + threadIdx.x, j = A.lo[1] + B.y * 4 + threadIdx.y;
+    const int k = A.lo[2] + B.z;
     if (i >= A.hi[0] || j >= A.hi[1] || k >= A.hi[2]) return;
     const Geo& g = A.g;
     const long long L = gidx(g, i, j, k);
@@ -437,25 +422,16 @@ void launch_stage(const StageArgsT<S>& a, hipStream_t st) {
 template <typename S>
 void launch_bc108(const BCArgsT<S>& a, hipStream_t st) {
     const Geo& g = a.g;
-    hipLaunchKernelGGL(k_bc_inlet<S>, dim3((g.ny + 63) / 64, g.nz), dim3(64), 0, st, a);
-    if (g.nx - a.x_begin > 0)
-        hipLaunchKernelGGL(k_bc_top<S>, dim3((g.nx - a.x_begin + 63) / 64, g.nz), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(k_bc_wall_vel<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
-    if (a.wall_not_normal == 0) hipLaunchKernelGGL(k_bc_wall_T<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
+    const int nt = g.ny + (g.nx - a.x_begin > 0 ? g.nx - a.x_begin : 0);
+    hipLaunchKernelGGL(k_bc_inlet_top<S>, dim3((nt + 63) / 64, g.nz), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc_wall<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
 }
 
 template <typename S>
 void launch_face_refresh(const BCArgsT<S>& a, hipStream_t st) {
     const Geo& g = a.g;
-    if (a.amu) {
-        const int nt = g.ny + 2 * g.nx;
-        hipLaunchKernelGGL(k_bc_amu<S>, dim3((nt + 63) / 64, g.nz), dim3(64), 0, st, a);
-    }
-    for (int dir = 0; dir < 3; dir++) {
-        const int na = dir == 0 ? g.ny : g.nx, nb = dir == 2 ? g.ny : g.nz;
-        const long long tot = 2LL * na * nb;
-        hipLaunchKernelGGL(k_face_refresh<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, dir);
-    }
+    const long long tot = 2LL * g.ny * g.nz + 2LL * g.nx * g.nz + 2LL * g.nx * g.ny;
+    hipLaunchKernelGGL(k_faces<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
 }
 
 template <typename S>

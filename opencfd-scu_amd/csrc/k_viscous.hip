@@ -68,14 +68,17 @@ __device__ __forceinline__ R cd9(const R* s, int i, int n, int bl, int br, R rh)
 
 // Tile bookkeeping shared by both kernels.
 struct Tile25 {
-    int i, j, kbeg, kend;
+    int i0, j0, i, j, kbeg, kend;
     bool own;
 };
 __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
     Tile25 t;
-    t.i = blockIdx.x * TX + threadIdx.x;
-    t.j = blockIdx.y * TY + threadIdx.y;
-    t.kbeg = blockIdx.z * KZ;
+    const Blk B = xcd_block();
+    t.i0 = B.x * TX;
+    t.j0 = B.y * TY;
+    t.i = t.i0 + threadIdx.x;
+    t.j = t.j0 + threadIdx.y;
+    t.kbeg = B.z * KZ;
     t.kend = min(t.kbeg + KZ, g.nz);
     t.own = t.i < g.nx && t.j < g.ny;
     return t;
@@ -93,7 +96,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
     const Geo& g = A.g;
     const Tile25 t = tile_of(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
     __shared__ S pl[4][PN];
     const S* src[4] = {A.u, A.v, A.w, A.T};
@@ -224,7 +227,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
     const Geo& g = A.g;
     const Tile25 t = tile_of(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
     __shared__ S ex[4][XN];
     __shared__ S ey[4][YN];

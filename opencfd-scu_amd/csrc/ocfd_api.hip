@@ -450,8 +450,8 @@ int one_step_t(ocfd_ctx* x) {
     std::swap(x->f, x->fn);
     const bool fdue = x->flap && filter_due(x, x->istep + 1);
     for (int krk = 1; krk <= 3; krk++) {
-        TRY(rhs<S>(x, krk));
         const StageArgsT<S> sa = stage_args<S>(x, krk, krk == 3 && fdue);
+        TRY(rhs<S>(x, krk));
         TRY(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
         TRY(apply_bc<S>(x));
     }

@@ -26,6 +26,32 @@ __host__ __device__ __forceinline__ long long gidx(const Geo& g, int i, int j, i
     return g.off0 + (long long)i + (long long)g.ldx * ((long long)j + (long long)g.ldy * (long long)k);
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (each with its own L2), so consecutive block indices land on
+// different L2s and neighbouring tiles - which share 128-byte lines and halo
+// rows - would each fetch them from HBM.  xcd_block() maps the hardware block
+// index to a logical tile index such that every XCD walks one contiguous run
+// of tiles (x fastest): neighbours then share that XCD's L2.  It is a
+// permutation of [0, nblocks) whatever the hardware mapping is.
+struct Blk { int x, y, z; };
+__device__ __forceinline__ Blk xcd_block() {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned nb = gx * gy * gridDim.z;
+    const unsigned b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+#ifdef OCFD_NO_XCD_REMAP
+    const unsigned t = b;  // hardware order (for comparison builds)
+    (void)nb;
+#else
+    const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, s = b >> 3;
+    const unsigned t = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + s;
+#endif
+    Blk o;
+    o.x = (int)(t % gx);
+    o.y = (int)((t / gx) % gy);
+    o.z = (int)(t / (gx * gy));
+    return o;
+}
+
 // The 10 metric fields kept on the device (cofactors and Ajac,
 // src/OCFD_Comput_Jacobian3d.cu:285-295).
 enum { M_KX = 0, M_KY, M_KZ, M_IX, M_IY, M_IZ, M_SX, M_SY, M_SZ, M_JAC, M_COUNT };

@@ -64,7 +64,41 @@ struct StageArgsT {
     long long fs;
 };
 template <typename S> void launch_stage(const StageArgsT<S>& a, hipStream_t st);
-// cons->prim + p from f into the given primitive arrays (any storage type).
+
+// SSP-RK3 stage of one point (OCFD_time_advance_ker1/2/3, src/OCFD_time.cu:
+// 44-94) with cons->prim + p (cons_to_pri_kernel, src/commen_kernel.cu:
+// 77-109) and Sutherland's law (get_Amu_kernal, src/commen_kernel.cu:122-131);
+// du[m] is the point's right-hand side.
+template <typename S>
+__device__ __forceinline__ void stage_point(const StageArgsT<S>& A, long long q, const double* du) {
+    const long long fs = A.fs;
+    const double dt = A.dt;
+    double fv[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        const long long o = m * fs + q;
+        double r;
+        if (A.krk == 1) {
+            r = A.fn[o] + dt * du[m];
+        } else if (A.krk == 2) {
+            const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du[m]);
+        } else {
+            const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
+            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du[m]);
+        }
+        A.f[o] = r;
+        if (A.flap) A.flap[o] = r;
+        fv[m] = r;
+    }
+    const double d1 = fv[0];
+    const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
+    const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
+    A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
+    A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
+    A.amu[q] = S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2));
+}
+
 template <typename S> void launch_c2p_all(const StageArgsT<S>& a, hipStream_t st);
 
 // IBC 108 and the face refresh.
