@@ -19,7 +19,8 @@
 // into du are double.
 //
 // Both use 2.5-D blocking: a block owns a TX x TY tile of (x,y) columns and
-// marches through a chunk of z planes.  x and y neighbours come from the
+// marches through a chunk of z planes (32 on the FP64 path, 16 on the FP32
+// path, which then has twice the blocks).  x and y neighbours come from the
 // current plane staged in LDS with its 4-point halo; z neighbours come from a
 // 9-plane queue each thread keeps in registers for its own column.
 #include "ocfd_dev.h"
@@ -27,8 +28,11 @@
 
 namespace {
 
-#ifndef OCFD_VISC_KZ
-#define OCFD_VISC_KZ 16  // z planes per block
+#ifndef OCFD_VISC_KZ_D
+#define OCFD_VISC_KZ_D 32  // z planes per block, FP64 path
+#endif
+#ifndef OCFD_VISC_KZ_F
+#define OCFD_VISC_KZ_F 16  // z planes per block, FP32 path (more blocks)
 #endif
 #ifndef OCFD_VISC_TX
 #define OCFD_VISC_TX 32  // x columns per block
@@ -36,7 +40,13 @@ namespace {
 #ifndef OCFD_VISC_TY
 #define OCFD_VISC_TY 8  // y columns per block
 #endif
-constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY, KZ = OCFD_VISC_KZ;
+#ifndef OCFD_VISC_PREFETCH
+#define OCFD_VISC_PREFETCH 1  // fetch plane k+1 while plane k is computed
+#endif
+constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
+template <typename S> struct KZ_ { static constexpr int v = OCFD_VISC_KZ_D; };
+template <> struct KZ_<float> { static constexpr int v = OCFD_VISC_KZ_F; };
+constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
 
 // Central derivative at interior index i (0..n-1) from s[0..8] = values at
 // i-4..i+4 (OCFD_CD6/CD8_kernel + D0 closures + put_d0_kernel).  rh = 1/h; the
@@ -71,7 +81,9 @@ struct Tile25 {
     int i0, j0, i, j, kbeg, kend;
     bool own;
 };
+template <typename S>
 __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
+    constexpr int KZ = KZ_<S>::v;
     Tile25 t;
     const Blk B = xcd_block();
     t.i0 = B.x * TX;
@@ -94,7 +106,7 @@ template <int SV, typename S>
 __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
     constexpr int PW = TX + 8, PN = PW * (TY + 8), NSL = (PN + NT - 1) / NT;
     const Geo& g = A.g;
-    const Tile25 t = tile_of(g);
+    const Tile25 t = tile_of<S>(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
@@ -137,6 +149,11 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
     for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(t.kbeg + 4) * g.sz];
     load_plane(t.kbeg);
     for (int k = t.kbeg; k < t.kend; k++) {
+        if (!VISC_PREFETCH && k > t.kbeg) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(k + 4) * g.sz];
+            load_plane(k);
+        }
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             q[f][8] = nq[f];
@@ -146,7 +163,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
                 if (sl[r] >= 0) pl[f][sl[r]] = hv[r][f];
         }
         __syncthreads();
-        if (k + 1 < t.kend) {
+        if (VISC_PREFETCH && k + 1 < t.kend) {
 #pragma unroll
             for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(k + 5) * g.sz];
             load_plane(k + 1);
@@ -220,21 +237,24 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
 }
 
 // Ev_x (4 fields) needs x neighbours, Ev_y y neighbours, Ev_z z neighbours.
-template <int SV, typename S>
-__global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
+// One launch handles NC of the four flux components, starting at c0 (each
+// component is independent; fewer components per launch need fewer
+// registers).
+template <int SV, typename S, int NC>
+__global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     constexpr int XW = TX + 8, XN = TY * XW, YN = (TY + 8) * TX;
     constexpr int NSX = (XN + NT - 1) / NT, NSY = (YN + NT - 1) / NT;
     const Geo& g = A.g;
-    const Tile25 t = tile_of(g);
+    const Tile25 t = tile_of<S>(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
-    __shared__ S ex[4][XN];
-    __shared__ S ey[4][YN];
+    __shared__ S ex[NC][XN];
+    __shared__ S ey[NC][YN];
     const long long fs = A.fs;
-    const S* EX = A.ev;
-    const S* EY = A.ev + 4 * fs;
-    const S* EZ = A.ev + 8 * fs;
+    const S* EX = A.ev + c0 * fs;
+    const S* EY = A.ev + (4 + c0) * fs;
+    const S* EZ = A.ev + (8 + c0) * fs;
     const int ic = min(t.i, g.nx - 1), jc = min(t.j, g.ny - 1);
     const long long col = gidx(g, ic, jc, 0);
     int slx[NSX], sly[NSY];
@@ -255,29 +275,34 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
         sly[r] = p < YN ? p : -1;
         soy[r] = (gi < g.nx && gj <= g.ny + 3) ? gidx(g, gi, gj, 0) : -1;
     }
-    S xv[NSX][4], yv[NSY][4];
+    S xv[NSX][NC], yv[NSY][NC];
     auto load_plane = [&](int k) {
         const long long pk = (long long)k * g.sz;
 #pragma unroll
         for (int r = 0; r < NSX; r++)
 #pragma unroll
-            for (int f = 0; f < 4; f++) xv[r][f] = (slx[r] >= 0 && sox[r] >= 0) ? EX[f * fs + sox[r] + pk] : S(0);
+            for (int f = 0; f < NC; f++) xv[r][f] = (slx[r] >= 0 && sox[r] >= 0) ? EX[f * fs + sox[r] + pk] : S(0);
 #pragma unroll
         for (int r = 0; r < NSY; r++)
 #pragma unroll
-            for (int f = 0; f < 4; f++) yv[r][f] = (sly[r] >= 0 && soy[r] >= 0) ? EY[f * fs + soy[r] + pk] : S(0);
+            for (int f = 0; f < NC; f++) yv[r][f] = (sly[r] >= 0 && soy[r] >= 0) ? EY[f * fs + soy[r] + pk] : S(0);
     };
-    S q[4][9], nq[4];
+    S q[NC][9], nq[NC];
 #pragma unroll
     for (int m = 0; m < 8; m++)
 #pragma unroll
-        for (int f = 0; f < 4; f++) q[f][m] = EZ[f * fs + col + (long long)(t.kbeg - 4 + m) * g.sz];
+        for (int f = 0; f < NC; f++) q[f][m] = EZ[f * fs + col + (long long)(t.kbeg - 4 + m) * g.sz];
 #pragma unroll
-    for (int f = 0; f < 4; f++) nq[f] = EZ[f * fs + col + (long long)(t.kbeg + 4) * g.sz];
+    for (int f = 0; f < NC; f++) nq[f] = EZ[f * fs + col + (long long)(t.kbeg + 4) * g.sz];
     load_plane(t.kbeg);
     for (int k = t.kbeg; k < t.kend; k++) {
+        if (!VISC_PREFETCH && k > t.kbeg) {
 #pragma unroll
-        for (int f = 0; f < 4; f++) {
+            for (int f = 0; f < NC; f++) nq[f] = EZ[f * fs + col + (long long)(k + 4) * g.sz];
+            load_plane(k);
+        }
+#pragma unroll
+        for (int f = 0; f < NC; f++) {
             q[f][8] = nq[f];
 #pragma unroll
             for (int r = 0; r < NSX; r++)
@@ -287,16 +312,16 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
                 if (sly[r] >= 0) ey[f][sly[r]] = yv[r][f];
         }
         __syncthreads();
-        if (k + 1 < t.kend) {
+        if (VISC_PREFETCH && k + 1 < t.kend) {
 #pragma unroll
-            for (int f = 0; f < 4; f++) nq[f] = EZ[f * fs + col + (long long)(k + 5) * g.sz];
+            for (int f = 0; f < NC; f++) nq[f] = EZ[f * fs + col + (long long)(k + 5) * g.sz];
             load_plane(k + 1);
         }
         if (t.own) {
             const long long qi = gidx(g, t.i, t.j, k);
             const double ajac = A.M[M_JAC][qi];
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
+            for (int m = 0; m < NC; m++) {
                 S s[9];
 #pragma unroll
                 for (int r = 0; r < 9; r++) s[r] = ex[m][ty * XW + tx + r];
@@ -306,7 +331,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
                 const S vy = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
                 const S vz = cd9<SV>(q[m], k, g.nz, A.bnd[4], A.bnd[5], rhz);
                 // the Jacobian factor and the accumulation into du are double
-                S* dv = A.du + (long long)(m + 1) * fs + qi;
+                S* dv = A.du + (long long)(c0 + m + 1) * fs + qi;
                 double acc = double(*dv);
                 acc = acc + ajac * double(vx);
                 acc = acc + ajac * double(vy);
@@ -315,26 +340,38 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A) {
             }
         }
 #pragma unroll
-        for (int f = 0; f < 4; f++)
+        for (int f = 0; f < NC; f++)
 #pragma unroll
             for (int m = 0; m < 8; m++) q[f][m] = q[f][m + 1];
         __syncthreads();
     }
 }
 
-dim3 grid25(const Geo& g) { return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + KZ - 1) / KZ); }
+template <typename S>
+dim3 grid25(const Geo& g) {
+    constexpr int KZ = KZ_<S>::v;
+    return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + KZ - 1) / KZ);
+}
 
 }  // namespace
 
 template <typename S>
 void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), grid25(a.g), dim3(TX, TY), 0, st, a);
-    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), grid25(a.g), dim3(TX, TY), 0, st, a);
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), grid25<S>(a.g), dim3(TX, TY), 0, st, a);
+    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), grid25<S>(a.g), dim3(TX, TY), 0, st, a);
 }
+// Flux components per divergence launch: one on the FP64 path (a quarter of
+// the registers: 4 waves per SIMD instead of 2, measured faster), all four on
+// the FP32 path.
+template <typename S> struct NC_ { static constexpr int v = 1; };
+template <> struct NC_<float> { static constexpr int v = 4; };
 template <typename S>
 void launch_visc_div(const ViscArgsT<S>& a, hipStream_t st) {
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S>), grid25(a.g), dim3(TX, TY), 0, st, a);
-    else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S>), grid25(a.g), dim3(TX, TY), 0, st, a);
+    constexpr int NC = NC_<S>::v;
+    for (int c0 = 0; c0 < 4; c0 += NC) {
+        if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
+        else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
+    }
 }
 template void launch_visc_gf<double>(const ViscArgsT<double>&, hipStream_t);
 template void launch_visc_gf<float>(const ViscArgsT<float>&, hipStream_t);
