@@ -366,14 +366,16 @@ void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
 template <typename S> struct NC_ { static constexpr int v = 1; };
 template <> struct NC_<float> { static constexpr int v = 4; };
 template <typename S>
-void launch_visc_div(const ViscArgsT<S>& a, hipStream_t st) {
+int visc_div_components() { return NC_<S>::v; }
+template <typename S>
+void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st) {
     constexpr int NC = NC_<S>::v;
-    for (int c0 = 0; c0 < 4; c0 += NC) {
-        if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
-        else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
-    }
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
+    else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
 }
 template void launch_visc_gf<double>(const ViscArgsT<double>&, hipStream_t);
 template void launch_visc_gf<float>(const ViscArgsT<float>&, hipStream_t);
-template void launch_visc_div<double>(const ViscArgsT<double>&, hipStream_t);
-template void launch_visc_div<float>(const ViscArgsT<float>&, hipStream_t);
+template void launch_visc_div<double>(const ViscArgsT<double>&, int, hipStream_t);
+template void launch_visc_div<float>(const ViscArgsT<float>&, int, hipStream_t);
+template int visc_div_components<double>();
+template int visc_div_components<float>();

@@ -379,7 +379,8 @@ int rhs(ocfd_ctx* x, int krk) {
     for (int dir = 0; dir < 3; dir++)
         if (C.iperiodic[dir])
             TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
-    TRY(timed(x, "visc_div", [&] { launch_visc_div(va, x->st); }));
+    for (int c0 = 0; c0 < 4; c0 += visc_div_components<S>())
+        TRY(timed(x, "visc_div", [&] { launch_visc_div(va, c0, x->st); }));
     return 0;
 }
 

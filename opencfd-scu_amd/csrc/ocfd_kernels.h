@@ -42,7 +42,10 @@ struct ViscArgsT {
 };
 // Gradients of u,v,w,T (kept in registers) and the 12 viscous fluxes.
 template <typename S> void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st);
-template <typename S> void launch_visc_div(const ViscArgsT<S>& a, hipStream_t st);
+// The divergence in launches of visc_div_components<S>() flux components
+// each, starting at component c0 (0..3).
+template <typename S> int visc_div_components();
+template <typename S> void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st);
 
 // Periodic face fill (exchange_boundary_?_packed_dev at one rank): copy the
 // 4-layer faces of `nf` fields (stride fs) in direction dir.
