@@ -40,6 +40,11 @@ ALGO_BYTES = {
 }
 
 
+def log(msg):
+    """Progress on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +53,9 @@ def parse():
     ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--sweep", type=float, default=0.0, help="ramp sweep angle in degrees (30 for the swept ramp)")
+    ap.add_argument("--sensor-params", default=str(REPO / "opencfd-scu_amd" / "pyocfd" / "sensor_params.npy"),
+                    help="learned-sensor weights for the mixed + learned-sensor leg (skipped if absent)")
     ap.add_argument("--cpu-sample-steps", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
@@ -97,17 +105,20 @@ def cpu_baseline(case, steps):
                       f"C oracle with {threads} OpenMP threads, {dt:.1f} s"}
 
 
-def mixed_leg(case, solver64, args, value64):
+def mixed_leg(case, solver64, args, value64, sensor=None):
     """The same workload on the mixed-precision path (FP32 bulk; FP64 metrics,
     WENO weights, Jacobian factors and RK state) from the same initial state
-    and for the same warmup + timed steps: its rate, its speedup over the
-    FP64 path and the relative L2 error of density and pressure against the
-    FP64 path's state at the same step."""
+    and for the same warmup + timed steps, optionally with the learned hybrid
+    sensor: its rate, its speedup over the FP64 path and the relative L2
+    error of density and pressure against the FP64 path's state at the same
+    step."""
     import numpy as np
     from pyocfd import capi
     cfg = capi.OcfdConfig.from_buffer_copy(case.cfg)
     cfg.precision = capi.PREC_MIXED
     m = case.load(capi.Solver(cfg, device=0))
+    if sensor is not None:
+        m.set_sensor(sensor, 0.2, 0.8)
     if args.warmup:
         m.step(args.warmup)
     m.sync()
@@ -130,6 +141,11 @@ def mixed_leg(case, solver64, args, value64):
            "l2_rel_rho": l2(p32[0], p64[0]), "l2_rel_p": l2(pr32, pr64),
            "max_rel_rho": float(np.abs(p32[0] - p64[0]).max() / np.abs(p64[0]).max()),
            "after_steps": int(i64), "dtype": "f32 bulk / f64 metrics, weights, RK state"}
+    if sensor is not None:
+        flags = m.get_flags()
+        n = sum(f.size for f in flags)
+        out["sensor"] = "learned (BF16-MFMA MLP, confidence band [0.2, 0.8])"
+        out["weno_fraction"] = round(sum(int((f > 1).sum()) for f in flags) / n, 4)
     m.close()
     return out
 
@@ -143,11 +159,15 @@ def main():
         print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
               "(the multi-GPU halo exchange is a design only)", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     from pyocfd import capi, cases
 
-    workload = f"ramp{args.nx}x{args.ny}x{args.nz}"
-    case = cases.ramp(args.nx, args.ny, args.nz, nsteps=args.warmup + args.steps)
+    workload = f"ramp{args.nx}x{args.ny}x{args.nz}" + (f"_sweep{args.sweep:g}" if args.sweep else "")
+    log(f"generating {workload}")
+    case = cases.ramp(args.nx, args.ny, args.nz, sweep=args.sweep, nsteps=args.warmup + args.steps)
+    log("loading the FP64 context")
     solver = case.load(capi.Solver(case.cfg, device=0))
+    log("timing the FP64 path")
     npts = case.npoints
     if args.warmup:
         solver.step(args.warmup)
@@ -177,13 +197,17 @@ def main():
                 "avg_launch_ms": round(avg_s * 1e3, 4)}
     share = {k: round(v[1] / sum(x[1] for x in prof.values()), 4) for k, v in sorted(prof.items())}
     flags = solver.get_flags()
-    import numpy as np
     cnt = np.bincount(np.concatenate([f.ravel() for f in flags]), minlength=4)[1:4]
-    mixed = None
+    mixed = learned = None
     if not args.no_mixed:
+        log("timing the mixed-precision path")
         mixed = mixed_leg(case, solver, args, value)
+        if Path(args.sensor_params).exists():
+            log("timing the mixed-precision path with the learned sensor")
+            learned = mixed_leg(case, solver, args, value, sensor=np.load(args.sensor_params))
     cpu = None
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
+        log("timing the CPU oracle")
         cpu = cpu_baseline(case, args.cpu_sample_steps)
     out = {
         "metric": "grid-point RK-stage updates/sec on 1 MI355X",
@@ -197,7 +221,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generator: tanh-stretched 24-degree ramp, Blasius-like inflow)",
+        "data": "synthetic (seeded generator: tanh-stretched 24-degree ramp%s, Blasius-like inflow)" % (
+            f", swept {args.sweep:g} degrees" if args.sweep else ""),
         "config": {"workload": workload, "points": npts, "scheme": "hybrid_auto WENO7/UP7 + CD8",
                    "ibc": 108, "filter_every": 10, "precision": "fp64",
                    "hybrid_flag_fractions": {"UP7": round(cnt[0] / cnt.sum(), 4),
@@ -207,6 +232,7 @@ def main():
         "kernel_share": share,
         "roofline": roof,
         "mixed_precision": mixed,
+        "mixed_learned_sensor": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -825,15 +825,11 @@ int ocfd_get_flow(ocfd_ctx* x, double* d, double* u, double* v, double* w, doubl
     if (!x) return fail(OCFD_ERR_ARG, "null context");
     double* dst[5] = {d, u, v, w, T};
     if (x->mixed) {
-        // FP64 primitives from the FP64 conserved state (get_duvwT arithmetic)
-        // into the scratch fields
-        StageArgsT<double> sa;
-        sa.g = x->g; sa.c = x->c; sa.f = x->f; sa.fs = x->fs;
-        sa.d = x->grad; sa.u = x->grad + x->fs; sa.v = x->grad + 2 * x->fs; sa.w = x->grad + 3 * x->fs;
-        sa.T = x->grad + 4 * x->fs; sa.P = x->grad + 5 * x->fs;
-        TRY(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
+        // the stored FP32 primitives: like the reference's they lag the
+        // filter, which changes only f at the end of a step
+        const float* src[5] = {x->sd, x->su, x->sv, x->sw, x->sT};
         for (int q = 0; q < 5; q++)
-            if (dst[q]) TRY(download_interior(x, x->grad + q * x->fs, dst[q]));
+            if (dst[q]) TRY(download_any(x, src[q], dst[q]));
     } else {
         const double* src[5] = {x->d, x->u, x->v, x->w, x->T};
         for (int q = 0; q < 5; q++)

@@ -268,15 +268,62 @@ def blasius_like(yn, delta, ma, gamma, tw, pr=0.72):
     return d, U, T
 
 
+def oblique_shock(ma, theta_deg, gamma=1.4):
+    """Weak oblique-shock solution for a wedge of theta: (beta, rho2/rho1,
+    p2/p1, u2, v2) with the upstream velocity (1, 0)."""
+    th = math.radians(theta_deg)
+
+    def f(b):
+        return 2 / math.tan(b) * (ma * ma * math.sin(b) ** 2 - 1) / (ma * ma * (gamma + math.cos(2 * b)) + 2) - \
+            math.tan(th)
+
+    lo, hi = math.asin(1.0 / ma) + 1e-9, math.radians(65.0)  # weak branch below the max-deflection angle
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid) < 0:
+            lo = mid
+        else:
+            hi = mid
+    b = 0.5 * (lo + hi)
+    mn2 = (ma * math.sin(b)) ** 2
+    r21 = (gamma + 1) * mn2 / ((gamma - 1) * mn2 + 2)
+    p21 = 1 + 2 * gamma / (gamma + 1) * (mn2 - 1)
+    ut, un = math.cos(b), math.sin(b) / r21
+    u2 = ut * math.cos(b) + un * math.sin(b)
+    v2 = ut * math.sin(b) - un * math.cos(b)
+    return b, r21, p21, u2, v2
+
+
 def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=0.02, tw=2.5,
          pert=0.01, seed=SEED, cfl=0.5, scheme="SCHEME_HYBRIDAUTO", filt=True, filter_every=10,
-         nsteps=1000, precision=capi.PREC_FP64, height=0.5, span=0.1, t_ref=108.1) -> Case:
+         nsteps=1000, precision=capi.PREC_FP64, height=0.5, span=0.1, t_ref=108.1, shock=True,
+         corner=0.5) -> Case:
+    """Compression ramp.  With shock=True the initial inviscid field already
+    holds the weak oblique shock of the ramp (from the corner at the
+    theta-beta-Mach angle, smoothed over two cells) with the post-shock state
+    turned parallel to the ramp, under the boundary-layer profile; upstream
+    and at the inflow it is the freestream."""
     gamma = 1.4
-    x, y, z, ycol = ramp_mesh(nx, ny, nz, angle, sweep, height=height, span=span)
+    x, y, z, ycol = ramp_mesh(nx, ny, nz, angle, sweep, height=height, span=span, corner=corner)
     # wall distance along each grid line (vertical lines)
     yn = y - y[:, :1, :]
     d1, U1, T1 = blasius_like(ycol, delta, ma, gamma, tw)
     d, U, T = blasius_like(yn, delta, ma, gamma, tw)
+    ui, vi = 1.0, 0.0
+    if shock:
+        beta, r21, p21, u2, v2 = oblique_shock(ma, angle, gamma)
+        xs = x - z * math.tan(math.radians(sweep))  # streamwise coordinate of the unswept section
+        dist = (y - (xs - corner) * math.tan(beta)) * math.cos(beta)
+        ws = 0.5 * (1.0 - np.tanh(dist / (2.0 / (nx - 1))))
+        rho_i = 1.0 + (r21 - 1.0) * ws
+        p_i = (1.0 + (p21 - 1.0) * ws) / (gamma * ma * ma)
+        ui = 1.0 + (u2 - 1.0) * ws
+        vi = v2 * ws
+        T_i = gamma * ma * ma * p_i / rho_i
+        # the boundary-layer temperature profile runs from the wall value to
+        # the local inviscid temperature
+        T = tw + (T - tw) * (T_i - tw) / (1.0 - tw)
+        d = gamma * ma * ma * p_i / T
     # seeded perturbations: white noise smoothed over a few cells (periodic in
     # z), scaled to a peak of `pert`, with an envelope that vanishes at the
     # wall and peaks at y = delta
@@ -286,10 +333,13 @@ def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=
     env = eta * np.exp(1.0 - eta)
     comps = []
     for _ in range(3):
-        r = gaussian_filter(rng.standard_normal(U.shape), sigma=(2.0, 2.0, 3.0), mode=("wrap", "nearest", "nearest"))
-        comps.append(pert * r / max(np.abs(r).max(), 1e-300) * env)
-    u = U + comps[0]
-    v = comps[1]
+        # FP32 noise: the same statistics at half the memory and time on
+        # large grids
+        r = gaussian_filter(rng.standard_normal(U.shape, dtype=np.float32), sigma=(2.0, 2.0, 3.0),
+                            mode=("wrap", "nearest", "nearest"))
+        comps.append(pert * (r / max(float(np.abs(r).max()), 1e-30)).astype(np.float64) * env)
+    u = U * ui + comps[0]
+    v = U * vi + comps[1]
     w = comps[2]
     inlet = np.stack([d1, U1, np.zeros(ny), T1])
     # CFL from the inviscid spectral radius in computational space (estimate
