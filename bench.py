@@ -164,7 +164,8 @@ def main():
 
     workload = f"ramp{args.nx}x{args.ny}x{args.nz}" + (f"_sweep{args.sweep:g}" if args.sweep else "")
     log(f"generating {workload}")
-    case = cases.ramp(args.nx, args.ny, args.nz, sweep=args.sweep, nsteps=args.warmup + args.steps)
+    case = cases.ramp(args.nx, args.ny, args.nz, sweep=args.sweep, nsteps=args.warmup + args.steps,
+                      progress=lambda m: log(f"case: {m}"))
     log("loading the FP64 context")
     solver = case.load(capi.Solver(case.cfg, device=0))
     log("timing the FP64 path")

@@ -297,14 +297,17 @@ def oblique_shock(ma, theta_deg, gamma=1.4):
 def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=0.02, tw=2.5,
          pert=0.01, seed=SEED, cfl=0.5, scheme="SCHEME_HYBRIDAUTO", filt=True, filter_every=10,
          nsteps=1000, precision=capi.PREC_FP64, height=0.5, span=0.1, t_ref=108.1, shock=True,
-         corner=0.5) -> Case:
+         corner=0.5, progress=None) -> Case:
     """Compression ramp.  With shock=True the initial inviscid field already
     holds the weak oblique shock of the ramp (from the corner at the
     theta-beta-Mach angle, smoothed over two cells) with the post-shock state
     turned parallel to the ramp, under the boundary-layer profile; upstream
-    and at the inflow it is the freestream."""
+    and at the inflow it is the freestream.  progress(msg), if given, is
+    called after each stage of the generation (large grids take minutes)."""
+    say = progress or (lambda msg: None)
     gamma = 1.4
     x, y, z, ycol = ramp_mesh(nx, ny, nz, angle, sweep, height=height, span=span, corner=corner)
+    say("mesh")
     # wall distance along each grid line (vertical lines)
     yn = y - y[:, :1, :]
     d1, U1, T1 = blasius_like(ycol, delta, ma, gamma, tw)
@@ -338,6 +341,7 @@ def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=
         r = gaussian_filter(rng.standard_normal(U.shape, dtype=np.float32), sigma=(2.0, 2.0, 3.0),
                             mode=("wrap", "nearest", "nearest"))
         comps.append(pert * (r / max(float(np.abs(r).max()), 1e-30)).astype(np.float64) * env)
+        say(f"perturbation component {len(comps)} of 3")
     u = U * ui + comps[0]
     v = U * vi + comps[1]
     w = comps[2]
