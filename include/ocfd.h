@@ -30,6 +30,8 @@ extern "C" {
 #define OCFD_LAP 4          /* head/config_parameters.h:15 */
 #define OCFD_NVARS 5        /* head/config_parameters.h:17 */
 #define OCFD_MAX_FILTERS 10 /* src/parameters.c:102 (NF_max) */
+#define OCFD_MAX_ANA 16     /* ANA_EVENT& keys kept (the reference mallocs N_ana) */
+#define OCFD_ANA_NPARA 100  /* ANA_NPARA&/ANA_RPARA& values, src/parameters.c:346-358 */
 
 /* Scheme identifiers, src/parameters.c:567-637. */
 enum {
@@ -45,7 +47,8 @@ enum { OCFD_PREC_FP64 = 0, OCFD_PREC_MIXED = 1 };
 /* Status codes. */
 enum {
     OCFD_OK = 0, OCFD_ERR_ARG = 1, OCFD_ERR_CONFIG = 2, OCFD_ERR_IO = 3,
-    OCFD_ERR_HIP = 4, OCFD_ERR_UNSUPPORTED = 5, OCFD_ERR_STATE = 6
+    OCFD_ERR_HIP = 4, OCFD_ERR_UNSUPPORTED = 5, OCFD_ERR_STATE = 6,
+    OCFD_ERR_NUMERIC = 7 /* NaN or too many negative T (the reference aborts) */
 };
 
 /*
@@ -82,6 +85,12 @@ typedef struct ocfd_config {
     int filter_npara[OCFD_MAX_FILTERS][11]; /* FILTER_NPARA&                */
     double filter_rpara[OCFD_MAX_FILTERS][3]; /* FILTER_RPARA&              */
     int precision;               /* OCFD_PREC_* (not an input key)           */
+    int nana;                    /* number of ANA_EVENT& keys                */
+    int ana_event[OCFD_MAX_ANA][2];       /* ANA_EVENT&: style, every Kstep  */
+    int n_ana_npara[OCFD_MAX_ANA];        /* values given in ANA_NPARA&      */
+    int n_ana_rpara[OCFD_MAX_ANA];        /* values given in ANA_RPARA&      */
+    int ana_npara[OCFD_MAX_ANA][OCFD_ANA_NPARA];    /* ANA_NPARA&            */
+    double ana_rpara[OCFD_MAX_ANA][OCFD_ANA_NPARA]; /* ANA_RPARA&            */
 } ocfd_config;
 
 typedef struct ocfd_ctx ocfd_ctx;
@@ -92,7 +101,8 @@ typedef struct ocfd_ctx ocfd_ctx;
  * (src/parameters.c:197-564, 667-792): spaces in keys are ignored, the last
  * occurrence of a key wins, '#' lines never match, unknown keys are ignored,
  * numbered FILTER_NPARA&/FILTER_RPARA& families are discovered by stripping
- * digits.  Unset keys are 0.  Returns OCFD_ERR_CONFIG on a bad scheme name
+ * digits, and so are ANA_EVENT&/ANA_NPARA&/ANA_RPARA& (src/parameters.c:
+ * 328-358).  Unset keys are 0.  Returns OCFD_ERR_CONFIG on a bad scheme name
  * (the reference exits, src/parameters.c:661-664). */
 int ocfd_read_config(const char *path, ocfd_config *cfg);
 
@@ -179,6 +189,40 @@ int ocfd_write_jacobi(ocfd_ctx *ctx, const char *path);
 /* hybrid_auto flags: x (nx+1)*ny*nz, y nx*(ny+1)*nz, z nx*ny*(nz+1). */
 int ocfd_get_hybrid_flags(ocfd_ctx *ctx, int8_t *fx, int8_t *fy, int8_t *fz);
 
+/* ---- analysis events (OCFD_ana, src/OCFD_ana.cu:588-619) ---------------
+ * ocfd_analysis runs the event of `style` with the parameters of ANA_NPARA&
+ * number `id` of the configuration (the id-th ANA_EVENT& key), writing its
+ * files into the working directory as the reference does:
+ *   100  NaN / negative-T check of T over the LAP box (ana_NAN_and_NT,
+ *        src/OCFD_ana.cu:105-222): prints each location, returns
+ *        OCFD_ERR_NUMERIC on a NaN or on more than 10 negative T.
+ *   101  running time average of d,u,v,w,T (ana_time_average, 224-298): the
+ *        first call reads ./opencfd.average when present (src/OCFD_IO.c:
+ *        94-151), and OCFD%08d.average is written when Istep % KSTEP_SAVE == 0.
+ *   102  Scheme_{x,y,z}%08d.dat, the flags of the plane z = NZ/2
+ *        (HybridAuto_scheme_IO, src/OCFD_Schemes_hybrid_auto.cu:706-744).
+ *   103  Q criterion to Q.dat (records) and Q000000.dat (text) (get_Q,
+ *        src/OCFD_ana.cu:299-458; the reference then exits, which is the
+ *        caller's business here).
+ *   104  Savedata-XY%03d.dat, 105 Savedata-YZ%03d.dat, 106 Savedata-XZ%03d.dat:
+ *        planes of d,u,v,w,T appended (ana_saveplaneXY/YZ/XZ, 459-586).
+ *   107  fractions of the three hybrid scheme types per direction, printed
+ *        (HybridAuto_scheme_Proportion, src/OCFD_Schemes_hybrid_auto.cu:
+ *        747-830); nothing when the scheme is not hybrid_auto.
+ * Other styles are ignored, as in the reference's switch. */
+int ocfd_analysis(ocfd_ctx *ctx, int style, int id);
+/* Q criterion at the interior points (nz*ny*nx). */
+int ocfd_get_q(ocfd_ctx *ctx, double *q);
+/* The time averages of style 101 (interior, nz*ny*nx each), the number of
+ * samples in them and their accumulated time; OCFD_ERR_STATE before the
+ * first style-101 event. */
+int ocfd_get_time_average(ocfd_ctx *ctx, double *d, double *u, double *v,
+                          double *w, double *T, int *nsamples, double *tt_avg);
+/* The nine numbers style 107 prints: the fractions of type 1, 2, 3 flags in
+ * x, then y, then z, with the reference's normalisation (count over the
+ * direction's face array / (the two other global sizes) / (N + 1)). */
+int ocfd_hybrid_proportions(ocfd_ctx *ctx, double frac9[9]);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the
  * context's stream and accumulated per kernel name. */
 /* ---- learned hybrid sensor (no counterpart in the reference) ----------
@@ -238,7 +282,8 @@ void opencfd_mem_init_all(void);
  * ./OCFD3d-Mesh.dat, flow from Opencfd.msg/opencfd.dat, IBC set-up. */
 void init(void);
 /* NS_solver_real (src/OCFD_NS_Solver.cu:39): the time loop to END_TIME with
- * KSTEP_SAVE output files. */
+ * the ANA_EVENT analysis events, the KSTEP_SHOW diagnostics and the
+ * KSTEP_SAVE output files, in the reference's order. */
 void NS_solver_real(void);
 /* du_comput (src/OCFD_Stream.cu:28). */
 void du_comput(int KRK);
@@ -253,6 +298,9 @@ void OCFD_bc(void);
 /* filtering (src/OCFD_filtering.cu:75); the pointers are the reference's host
  * mirrors and are not dereferenced. */
 void filtering(double *pf, double *pf0, double *pp);
+/* OCFD_ana (src/OCFD_ana.cu:588): ocfd_analysis on the process-wide context;
+ * a failed style 100 aborts the process as MPI_Abort does. */
+void OCFD_ana(int style, int ID);
 /* opencfd_mem_finalize_all (src/OCFD_init.cu:238). */
 void opencfd_mem_finalize_all(void);
 /* Access to the process-wide context used by the functions above. */

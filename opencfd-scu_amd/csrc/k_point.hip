@@ -329,6 +329,26 @@ __global__ void __launch_bounds__(256) k_amu_all(BCArgsT<S> A) {
     A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
 }
 
+// Running time average of d, u, v, w, T over the interior
+// (ana_time_average_kernel, src/OCFD_ana.cu:240-254): m = (n m + x)/(n + 1)
+// with n the number of samples already in m.  avg holds 5 FP64 fields.
+template <typename S>
+__global__ void __launch_bounds__(256) k_time_avg(Geo g, const S* __restrict__ d, const S* __restrict__ u,
+                                                  const S* __restrict__ v, const S* __restrict__ w,
+                                                  const S* __restrict__ T, double* __restrict__ avg, long long fs,
+                                                  int n) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const long long q = gidx(g, i, j, k);
+    const S* src[5] = {d, u, v, w, T};
+    const double dn = double(n);
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        double* a = avg + m * fs + q;
+        *a = (dn * *a + double(src[m][q])) / (dn + 1.0);
+    }
+}
+
 __global__ void k_set(double* p, long long n, double v) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = v;
@@ -483,6 +503,12 @@ void launch_convert(To* dst, const From* src, long long n, hipStream_t st) {
     hipLaunchKernelGGL((k_convert<To, From>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
 }
 
+template <typename S>
+void launch_time_avg(const Geo& g, const S* d, const S* u, const S* v, const S* w, const S* T, double* avg,
+                     long long fs, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_time_avg<S>, pgrid(g), dim3(64, 4), 0, st, g, d, u, v, w, T, avg, fs, n);
+}
+
 int diag_sum_blocks(const Geo& g) { return ((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz; }
 
 template <typename S>
@@ -504,7 +530,9 @@ void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, 
     template void launch_p2c_all<S>(const BCArgsT<S>&, hipStream_t);                                         \
     template void launch_c2p_all<S>(const StageArgsT<S>&, hipStream_t);                                      \
     template void launch_amu_all<S>(const BCArgsT<S>&, hipStream_t);                                        \
-    template void launch_diag<S>(const Geo&, const double*, const S*, double*, double*, int*, hipStream_t);
+    template void launch_diag<S>(const Geo&, const double*, const S*, double*, double*, int*, hipStream_t);  \
+    template void launch_time_avg<S>(const Geo&, const S*, const S*, const S*, const S*, const S*, double*, \
+                                     long long, int, hipStream_t);
 OCFD_INSTANTIATE(double)
 OCFD_INSTANTIATE(float)
 #undef OCFD_INSTANTIATE

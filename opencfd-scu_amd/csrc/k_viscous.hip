@@ -13,6 +13,7 @@
 //   k_visc_div : du[1..4] += Ajac*d(Ev)/dxi, then eta, then zeta (OCFD_d?0 of
 //                Ev + YF_Pe_XF, src/OCFD_NS_Jacobian3d.cu:304-440,
 //                src/commen_kernel.cu:158-169), no atomics.
+//   k_qcrit    : the Q criterion of the analysis events (same derivatives).
 //
 // S (double or float) is the type of u,v,w,T, mu, the derivatives, the
 // fluxes and du; the Jacobian factor of the divergence and its accumulation
@@ -347,6 +348,50 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     }
 }
 
+// Q criterion of the velocity (get_Q / get_Q_kernal, src/OCFD_ana.cu:
+// 299-420): the nine derivatives of u, v, w along xi/eta/zeta with the
+// viscous scheme and its D0 closures (OCFD_dx0/dy0/dz0), the Cartesian
+// gradient from the cofactors, Q = (ux vy + ux wz + vy wz - uy vx - uz wx
+// - vz wy) Ajac^2.  A diagnostic run once: one thread per point, stencils read
+// straight from memory, FP64 arithmetic on both paths.
+template <int SV, typename S>
+__global__ void __launch_bounds__(256) k_qcrit(ViscArgsT<S> A, double* __restrict__ qout) {
+    const Geo& g = A.g;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const long long qi = gidx(g, i, j, k);
+    const double rhx = 1.0 / g.hx, rhy = 1.0 / g.hy, rhz = 1.0 / g.hz;
+    const S* src[3] = {A.u, A.v, A.w};
+    double gr[3][3];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        double s[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) s[m] = double(src[f][qi + (m - 4)]);
+        gr[f][0] = cd9<SV>(s, i, g.nx, A.bnd[0], A.bnd[1], rhx);
+#pragma unroll
+        for (int m = 0; m < 9; m++) s[m] = double(src[f][qi + (long long)(m - 4) * g.ldx]);
+        gr[f][1] = cd9<SV>(s, j, g.ny, A.bnd[2], A.bnd[3], rhy);
+#pragma unroll
+        for (int m = 0; m < 9; m++) s[m] = double(src[f][qi + (long long)(m - 4) * g.sz]);
+        gr[f][2] = cd9<SV>(s, k, g.nz, A.bnd[4], A.bnd[5], rhz);
+    }
+    const double Akx = A.M[M_KX][qi], Aky = A.M[M_KY][qi], Akz = A.M[M_KZ][qi];
+    const double Aix = A.M[M_IX][qi], Aiy = A.M[M_IY][qi], Aiz = A.M[M_IZ][qi];
+    const double Asx = A.M[M_SX][qi], Asy = A.M[M_SY][qi], Asz = A.M[M_SZ][qi];
+    const double ux = gr[0][0] * Akx + gr[0][1] * Aix + gr[0][2] * Asx;
+    const double vx = gr[1][0] * Akx + gr[1][1] * Aix + gr[1][2] * Asx;
+    const double wx = gr[2][0] * Akx + gr[2][1] * Aix + gr[2][2] * Asx;
+    const double uy = gr[0][0] * Aky + gr[0][1] * Aiy + gr[0][2] * Asy;
+    const double vy = gr[1][0] * Aky + gr[1][1] * Aiy + gr[1][2] * Asy;
+    const double wy = gr[2][0] * Aky + gr[2][1] * Aiy + gr[2][2] * Asy;
+    const double uz = gr[0][0] * Akz + gr[0][1] * Aiz + gr[0][2] * Asz;
+    const double vz = gr[1][0] * Akz + gr[1][1] * Aiz + gr[1][2] * Asz;
+    const double wz = gr[2][0] * Akz + gr[2][1] * Aiz + gr[2][2] * Asz;
+    const double aj = A.M[M_JAC][qi];
+    qout[qi] = (ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy) * aj * aj;
+}
+
 template <typename S>
 dim3 grid25(const Geo& g) {
     constexpr int KZ = KZ_<S>::v;
@@ -373,6 +418,14 @@ void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st) {
     if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
     else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
 }
+template <typename S>
+void launch_qcrit(const ViscArgsT<S>& a, double* q, hipStream_t st) {
+    const dim3 grid((a.g.nx + 63) / 64, (a.g.ny + 3) / 4, a.g.nz);
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_qcrit<OCFD_CD6, S>), grid, dim3(64, 4), 0, st, a, q);
+    else hipLaunchKernelGGL((k_qcrit<OCFD_CD8, S>), grid, dim3(64, 4), 0, st, a, q);
+}
+template void launch_qcrit<double>(const ViscArgsT<double>&, double*, hipStream_t);
+template void launch_qcrit<float>(const ViscArgsT<float>&, double*, hipStream_t);
 template void launch_visc_gf<double>(const ViscArgsT<double>&, hipStream_t);
 template void launch_visc_gf<float>(const ViscArgsT<float>&, hipStream_t);
 template void launch_visc_div<double>(const ViscArgsT<double>&, int, hipStream_t);

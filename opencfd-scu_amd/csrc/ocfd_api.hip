@@ -72,6 +72,9 @@ struct ocfd_ctx {
     float mlp_lo = 0.2f, mlp_hi = 0.8f;
     double* diag_part = nullptr;  // 2 * diag_sum_blocks partial sums
     int* diag_scan = nullptr;     // 2 * DIAG_SCAN_BLOCKS counts
+    double* avg = nullptr;        // time averages of d,u,v,w,T (analysis style 101)
+    int avg_n = 0;                // samples in them (Istep_average)
+    double avg_tt = 0.0;          // their accumulated time (tt_average)
     // host state
     std::vector<double> coord_x, coord_z;  // mesh x, z (interior, for IBC 108 set-up)
     std::vector<double> coord_top_x;       // x along the top row at k=0
@@ -974,6 +977,338 @@ int ocfd_get_hybrid_flags(ocfd_ctx* x, int8_t* fx, int8_t* fy, int8_t* fz) {
 }
 
 // ---------------------------------------------------------------------------
+// Analysis events (OCFD_ana, src/OCFD_ana.cu:588-619).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Whole LAP box of a field (index = gidx) as doubles.
+template <typename S>
+int download_box(ocfd_ctx* x, const S* src, std::vector<double>& out) {
+    std::vector<S> h((size_t)x->fs);
+    HIPCK(hipStreamSynchronize(x->st));
+    HIPCK(hipMemcpy(h.data(), src, sizeof(S) * (size_t)x->fs, hipMemcpyDeviceToHost));
+    out.assign(h.begin(), h.end());
+    return 0;
+}
+int download_box_any(ocfd_ctx* x, int field, std::vector<double>& out) {
+    // field: 0..4 = d,u,v,w,T
+    if (x->mixed) {
+        float* f[5] = {x->sd, x->su, x->sv, x->sw, x->sT};
+        return download_box(x, f[field], out);
+    }
+    double* f[5] = {x->d, x->u, x->v, x->w, x->T};
+    return download_box(x, f[field], out);
+}
+
+// ana_NAN_and_NT (src/OCFD_ana.cu:105-222): T over the LAP box.  The device
+// scan counts; the locations are printed from a host copy only when there is
+// something to print.
+int ana_nan_nt(ocfd_ctx* x) {
+    const Geo& g = x->g;
+    long long nnan = 0, nneg = 0;
+    TRY(ocfd_diagnostics(x, nullptr, nullptr, &nnan, &nneg));
+    std::printf("It is analyzing NAN......\n");
+    if (nnan || nneg) {
+        std::vector<double> T;
+        TRY(download_box_any(x, 4, T));
+        bool first = true;
+        for (int k = -4; k < g.nz + 4 && nnan && first; k++)
+            for (int j = -4; j < g.ny + 4 && first; j++)
+                for (int i = -4; i < g.nx + 4 && first; i++)
+                    if (std::isnan(T[(size_t)gidx(g, i, j, k)])) {
+                        std::printf("NAN occured in Global ID(%d , %d , %d)\n\n", i, j, k);
+                        first = false;
+                    }
+        for (int k = -4; k < g.nz + 4 && nneg; k++)
+            for (int j = -4; j < g.ny + 4; j++)
+                for (int i = -4; i < g.nx + 4; i++)
+                    if (T[(size_t)gidx(g, i, j, k)] < 0)
+                        std::printf("Negative T occured in Global ID(%d , %d , %d)\n\n", i, j, k);
+    }
+    if (nneg > 10) {
+        std::printf("Negative T points %lld > %d\n", nneg, 10);
+        return fail(OCFD_ERR_NUMERIC, "negative T at " + std::to_string(nneg) + " points");
+    }
+    if (nnan) {
+        std::printf("NAN occured , program Abort\n");
+        return fail(OCFD_ERR_NUMERIC, "NaN in T");
+    }
+    return 0;
+}
+
+// ana_time_average (src/OCFD_ana.cu:256-298) with the averaged-file read of
+// read_file (src/OCFD_IO.c:94-151) at the first call and OCFD_save(1, ...)
+// (src/OCFD_IO.c:156-214) every KSTEP_SAVE steps.
+int ana_time_average(ocfd_ctx* x) {
+    const Geo& g = x->g;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::printf("It is averaging......\n");
+    if (!x->avg) {
+        int rc = 0;
+        double* a = alloc_fields(x, 5, rc);  // zeroed: init_time_average
+        if (rc) return rc;
+        x->avg = a;
+        x->avg_n = 0;
+        x->avg_tt = 0.0;
+        const char* name = "opencfd.average";
+        FILE* fh = std::fopen(name, "rb");
+        if (!fh) {
+            std::printf("Average file: %s is not exit\n\n", name);
+        } else {
+            std::fclose(fh);
+            std::printf("read average_data begin\n");
+            std::vector<double> h(5 * n);
+            int is = 0;
+            double t = 0.0;
+            TRY(ocfd_read_flow_file(name, g.nx, g.ny, g.nz, h.data(), h.data() + n, h.data() + 2 * n,
+                                    h.data() + 3 * n, h.data() + 4 * n, &is, &t));
+            x->avg_n = is;
+            x->avg_tt = t;
+            std::printf("Istep_average=%d , tt_average=%lf\n", is, t);
+            for (int m = 0; m < 5; m++) TRY(upload_interior(x, x->avg + m * x->fs, h.data() + m * n, 0.0));
+            std::printf("read average_data ok\n");
+        }
+    }
+    if (x->mixed)
+        TRY(timed(x, "time_avg", [&] {
+            launch_time_avg(g, x->sd, x->su, x->sv, x->sw, x->sT, x->avg, x->fs, x->avg_n, x->st);
+        }));
+    else
+        TRY(timed(x, "time_avg", [&] {
+            launch_time_avg(g, x->d, x->u, x->v, x->w, x->T, x->avg, x->fs, x->avg_n, x->st);
+        }));
+    x->avg_n += 1;
+    x->avg_tt += x->cfg.dt;
+    if (x->cfg.kstep_save > 0 && x->istep % x->cfg.kstep_save == 0) {
+        std::vector<double> h(5 * n);
+        for (int m = 0; m < 5; m++) TRY(download_interior(x, x->avg + m * x->fs, h.data() + m * n));
+        char name[64];
+        std::snprintf(name, sizeof name, "OCFD%08d.average", x->avg_n);
+        std::printf("write data file: %s\n", name);
+        TRY(ocfd_write_flow_file(name, g.nx, g.ny, g.nz, h.data(), h.data() + n, h.data() + 2 * n, h.data() + 3 * n,
+                                 h.data() + 4 * n, x->avg_n, x->avg_tt));
+    }
+    return 0;
+}
+
+bool is_hybrid(const ocfd_ctx* x) { return x->cfg.scheme_invis == OCFD_HYBRIDAUTO && x->flag[0]; }
+
+// HybridAuto_scheme_IO (src/OCFD_Schemes_hybrid_auto.cu:706-744) through
+// write_2d_XY (src/OCFD_IO_mpi.c:63-125) at k = NZ/2: each line is the flag
+// at flat index (k, j, i+1) of the direction's face array read with the
+// interior row length of that array, and the pressure at node (i+1, j, k).
+// The reference prints an array that style 2 never writes (pPP_d); the
+// pressure p00*d*T that the style-2 sensor reads is printed instead.
+int ana_scheme_io(ocfd_ctx* x) {
+    if (!is_hybrid(x)) return fail(OCFD_ERR_STATE, "analysis style 102 needs SCHEME_HYBRIDAUTO");
+    const Geo& g = x->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz, K = nz / 2;
+    std::vector<int8_t> fl[3] = {std::vector<int8_t>((size_t)(nx + 1) * ny * nz),
+                                 std::vector<int8_t>((size_t)nx * (ny + 1) * nz),
+                                 std::vector<int8_t>((size_t)nx * ny * (nz + 1))};
+    TRY(ocfd_get_hybrid_flags(x, fl[0].data(), fl[1].data(), fl[2].data()));
+    std::vector<double> d, T;
+    TRY(download_box_any(x, 0, d));
+    TRY(download_box_any(x, 4, T));
+    const double p00 = x->c.p00;
+    const char dn[3] = {'x', 'y', 'z'};
+    for (int dd = 0; dd < 3; dd++) {
+        const size_t rowlen = dd == 0 ? nx + 1 : nx, rows = dd == 1 ? ny + 1 : ny;
+        char name[64];
+        std::snprintf(name, sizeof name, "Scheme_%c%08d.dat", dn[dd], x->istep);
+        FILE* fh = std::fopen(name, "w");
+        if (!fh) return fail(OCFD_ERR_IO, std::string("cannot write ") + name);
+        std::fprintf(fh, "variables=scheme\n");
+        std::fprintf(fh, "zone i=%d, j=%d\n", nx, ny);
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int flag = fl[dd][((size_t)K * rows + j) * rowlen + i + 1];
+                const size_t q = (size_t)gidx(g, i + 1, j, K);
+                std::fprintf(fh, "%08d%15.6lf\n", flag, p00 * d[q] * T[q]);
+            }
+        std::fclose(fh);
+    }
+    return 0;
+}
+
+// get_Q (src/OCFD_ana.cu:384-458): Q.dat as nz records of the interior, and
+// Q%02d%02d%02d.dat (rank 0 0 0) as text x y z Q.  The reference's kernel
+// writes the LAP positions of an array allocated without ghost layers; the
+// values it means, Q at the interior points, are what is written here.
+int ana_q(ocfd_ctx* x) {
+    const Geo& g = x->g;
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> q(n);
+    TRY(ocfd_get_q(x, q.data()));
+    TRY(ocfd_write_records("Q.dat", g.nx, g.ny, g.nz, 1, q.data()));
+    FILE* fh = std::fopen("Q000000.dat", "w");
+    if (!fh) return fail(OCFD_ERR_IO, "cannot write Q000000.dat");
+    std::fprintf(fh, "variables=x,y,z,Q\n");
+    std::fprintf(fh, "zone i=%d ,j=%d ,k=%d\n", g.nx, g.ny, g.nz);
+    const bool have = x->mesh.size() == 3 * n;
+    for (size_t p = 0; p < n; p++)
+        std::fprintf(fh, "%15.6f%15.6f%15.6f%15.6f\n", have ? x->mesh[p] : 0.0, have ? x->mesh[n + p] : 0.0,
+                     have ? x->mesh[2 * n + p] : 0.0, q[p]);
+    std::fclose(fh);
+    return 0;
+}
+
+// ana_saveplaneXY/YZ/XZ (src/OCFD_ana.cu:459-586) with write_2d_XYa/YZa/XZa
+// (src/OCFD_IO_mpi.c:128-260): per listed plane a header record {Istep, tt}
+// appended to Savedata-XY%03d.dat (YZ, XZ), then for each plane of the band
+// one record per variable d,u,v,w,T.  ANA_NPARA& = count, band width, first
+// plane of each band (0-based global indices).
+int ana_planes(ocfd_ctx* x, int style, int id) {
+    const ocfd_config& C = x->cfg;
+    const Geo& g = x->g;
+    if (id < 0 || id >= C.nana) return fail(OCFD_ERR_ARG, "analysis event id out of range");
+    const int* np = C.ana_npara[id];
+    const int count = np[0], band = np[1];
+    if (count < 0 || count + 2 > OCFD_ANA_NPARA || band < 0)
+        return fail(OCFD_ERR_CONFIG, "ANA_NPARA: bad plane count or band width");
+    const int dir = style == 104 ? 2 : style == 105 ? 0 : 1;  // the fixed index: z, x, y
+    const int nfix = dir == 0 ? g.nx : dir == 1 ? g.ny : g.nz;
+    for (int p = 0; p < count; p++)
+        if (np[p + 2] < 0 || np[p + 2] + band > nfix) return fail(OCFD_ERR_CONFIG, "ANA_NPARA: plane outside the grid");
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    std::vector<double> h(5 * n);
+    int is = 0;
+    double t = 0.0;
+    TRY(ocfd_get_flow(x, h.data(), h.data() + n, h.data() + 2 * n, h.data() + 3 * n, h.data() + 4 * n, &is, &t));
+    const char* tag = style == 104 ? "XY" : style == 105 ? "YZ" : "XZ";
+    for (int p = 0; p < count; p++) {
+        std::printf("Save data ...., %d, %lf, %d\n", x->istep, x->tt, p);
+        char name[64];
+        std::snprintf(name, sizeof name, "Savedata-%s%03d.dat", tag, p);
+        FILE* fh = std::fopen(name, "ab");
+        if (!fh) return fail(OCFD_ERR_IO, std::string("cannot write ") + name);
+        const int32_t hb = (int32_t)(sizeof(double) + sizeof(int));
+        const int32_t is32 = x->istep;
+        std::fwrite(&hb, 4, 1, fh);
+        std::fwrite(&is32, 4, 1, fh);
+        std::fwrite(&x->tt, 8, 1, fh);
+        std::fwrite(&hb, 4, 1, fh);
+        std::vector<double> plane;
+        for (int s = np[p + 2]; s < np[p + 2] + band; s++)
+            for (int m = 0; m < 5; m++) {
+                const double* f = h.data() + m * n;
+                plane.clear();
+                if (dir == 2) {  // XY: x fastest, then y
+                    plane.assign(f + (size_t)s * g.nx * g.ny, f + (size_t)(s + 1) * g.nx * g.ny);
+                } else if (dir == 0) {  // YZ: y fastest, then z
+                    for (int k = 0; k < g.nz; k++)
+                        for (int j = 0; j < g.ny; j++) plane.push_back(f[((size_t)k * g.ny + j) * g.nx + s]);
+                } else {  // XZ: x fastest, then z
+                    for (int k = 0; k < g.nz; k++)
+                        for (int i = 0; i < g.nx; i++) plane.push_back(f[((size_t)k * g.ny + s) * g.nx + i]);
+                }
+                const int32_t nb = (int32_t)(plane.size() * sizeof(double));
+                std::fwrite(&nb, 4, 1, fh);
+                std::fwrite(plane.data(), sizeof(double), plane.size(), fh);
+                std::fwrite(&nb, 4, 1, fh);
+            }
+        std::fclose(fh);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocfd_get_q(ocfd_ctx* x, double* q) {
+    if (!x || !q) return fail(OCFD_ERR_ARG, "null argument");
+    if (!x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
+    const ocfd_config& C = x->cfg;
+    double* out = x->grad;  // first scratch field
+    if (x->mixed) {
+        ViscArgsT<float> va;
+        va.g = x->g; va.c = x->c; va.svis = C.scheme_vis;
+        for (int m = 0; m < 6; m++) va.bnd[m] = C.dif_bound[m];
+        va.u = x->su; va.v = x->sv; va.w = x->sw; va.T = x->sT; va.amu = x->samu;
+        for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
+        va.ev = x->sev; va.du = x->sdu; va.fs = x->fs;
+        TRY(timed(x, "qcrit", [&] { launch_qcrit(va, out, x->st); }));
+    } else {
+        ViscArgsT<double> va;
+        va.g = x->g; va.c = x->c; va.svis = C.scheme_vis;
+        for (int m = 0; m < 6; m++) va.bnd[m] = C.dif_bound[m];
+        va.u = x->u; va.v = x->v; va.w = x->w; va.T = x->T; va.amu = x->amu;
+        for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
+        va.ev = x->ev; va.du = x->du; va.fs = x->fs;
+        TRY(timed(x, "qcrit", [&] { launch_qcrit(va, out, x->st); }));
+    }
+    return download_interior(x, out, q);
+}
+
+int ocfd_get_time_average(ocfd_ctx* x, double* d, double* u, double* v, double* w, double* T, int* nsamples,
+                          double* tt_avg) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    if (!x->avg) return fail(OCFD_ERR_STATE, "no time average yet (analysis style 101)");
+    double* outs[5] = {d, u, v, w, T};
+    for (int m = 0; m < 5; m++)
+        if (outs[m]) TRY(download_interior(x, x->avg + m * x->fs, outs[m]));
+    if (nsamples) *nsamples = x->avg_n;
+    if (tt_avg) *tt_avg = x->avg_tt;
+    return 0;
+}
+
+// HybridAuto_scheme_Proportion (src/OCFD_Schemes_hybrid_auto.cu:747-830):
+// counts of 1, 2 and anything else over each face array, divided by the
+// product of the two other global sizes and then by N + NP0 (NP0 = 1).
+int ocfd_hybrid_proportions(ocfd_ctx* x, double* frac9) {
+    if (!x || !frac9) return fail(OCFD_ERR_ARG, "null argument");
+    if (!is_hybrid(x)) return fail(OCFD_ERR_STATE, "hybrid proportions need SCHEME_HYBRIDAUTO");
+    const Geo& g = x->g;
+    const size_t sz[3] = {(size_t)(g.nx + 1) * g.ny * g.nz, (size_t)g.nx * (g.ny + 1) * g.nz,
+                          (size_t)g.nx * g.ny * (g.nz + 1)};
+    std::vector<int8_t> fl[3] = {std::vector<int8_t>(sz[0]), std::vector<int8_t>(sz[1]), std::vector<int8_t>(sz[2])};
+    TRY(ocfd_get_hybrid_flags(x, fl[0].data(), fl[1].data(), fl[2].data()));
+    const double other[3] = {(double)g.ny * g.nz, (double)g.nx * g.nz, (double)g.nx * g.ny};
+    const double len[3] = {(double)g.nx + 1, (double)g.ny + 1, (double)g.nz + 1};
+    for (int dd = 0; dd < 3; dd++) {
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        for (int8_t f : fl[dd]) {
+            if (f == 1) t1 += 1.0;
+            else if (f == 2) t2 += 1.0;
+            else t3 += 1.0;
+        }
+        frac9[3 * dd] = t1 / other[dd] / len[dd];
+        frac9[3 * dd + 1] = t2 / other[dd] / len[dd];
+        frac9[3 * dd + 2] = t3 / other[dd] / len[dd];
+    }
+    return 0;
+}
+
+int ocfd_analysis(ocfd_ctx* x, int style, int id) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    if (!x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
+    switch (style) {
+        case 100: return ana_nan_nt(x);
+        case 101: return ana_time_average(x);
+        case 102: return ana_scheme_io(x);
+        case 103: return ana_q(x);
+        case 104:
+        case 105:
+        case 106: return ana_planes(x, style, id);
+        case 107: {
+            if (!is_hybrid(x)) return 0;
+            double fr[9];
+            TRY(ocfd_hybrid_proportions(x, fr));
+            const char* dn[3] = {"X", "Y", "Z"};
+            for (int dd = 0; dd < 3; dd++)
+                std::printf("The first type scheme of Hybrid schemes in direction %s is %lf%%, second is %lf%%, "
+                            "third is %lf%%\n",
+                            dn[dd], fr[3 * dd], fr[3 * dd + 1], fr[3 * dd + 2]);
+            return 0;
+        }
+        default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Learned-sensor trainer: training samples, parameters and Adam moments on one
 // device.  No counterpart in the reference.
 // ---------------------------------------------------------------------------
@@ -1336,6 +1671,10 @@ void NS_solver_real(void) {
         SHIM(ocfd_step(g_ctx, 1));
         total += g_ctx->last_step_ms;
         window += g_ctx->last_step_ms;
+        // src/OCFD_NS_Solver.cu:99-101 (a zero Kstep is skipped: the reference
+        // would divide by it)
+        for (int i = 0; i < C.nana; i++)
+            if (C.ana_event[i][1] > 0 && g_ctx->istep % C.ana_event[i][1] == 0) OCFD_ana(C.ana_event[i][0], i);
         if (C.kstep_show > 0 && g_ctx->istep % C.kstep_show == 0) {
             // src/OCFD_NS_Solver.cu:124-152: mean total energy (NaN => the
             // NaN / negative-T check, which aborts) and mean T
@@ -1373,6 +1712,14 @@ void NS_solver_real(void) {
         if (C.end_time <= 0.0) break;
     } while (g_ctx->tt < C.end_time);
     std::printf("OK! opencfd is finished\nTotal GPU time %lf\n", total * 1e-3);
+}
+
+void OCFD_ana(int style, int ID) {
+    if (!g_ctx) die("OCFD_ana (call init first)", OCFD_ERR_STATE);
+    const int rc = ocfd_analysis(g_ctx, style, ID);
+    if (rc == OCFD_ERR_NUMERIC) std::exit(1);  // MPI_Abort, src/OCFD_ana.cu:212-221
+    if (rc) die(__func__, rc);
+    if (style == 103) std::exit(0);  // get_Q ends the run, src/OCFD_ana.cu:456
 }
 
 void du_comput(int KRK) { SHIM(rhs_any(g_ctx, KRK)); }

@@ -4,12 +4,13 @@
 //    197-564, 667-792).  A line is a candidate only when it holds '=' and ends
 //    with a newline; spaces are removed from the key; the last occurrence of a
 //    key wins; '#'-prefixed keys never match; unknown keys are ignored;
-//    FILTER_NPARA&/FILTER_RPARA& are numbered families found by stripping
-//    digits from the key.
+//    FILTER_NPARA&/FILTER_RPARA& and ANA_EVENT&/ANA_NPARA&/ANA_RPARA& are
+//    numbered families found by stripping digits from the key.
 //  * Fortran-record 3-D arrays: per z plane, int32 nbytes | nx*ny f64 |
 //    int32 nbytes (head/io_warp.h:6-20, src/OCFD_IO_mpi.c:562-740).
 //  * Flow files: header record {int32 Istep, f64 tt} then d,u,v,w,T
 //    (src/OCFD_IO.c:24-228).
+#include <algorithm>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -179,6 +180,26 @@ int ocfd_read_config(const char* path, ocfd_config* c) {
         }
     }
     c->nfilter = nf;
+    // ANA_EVENT& = style Kstep, with ANA_NPARA& / ANA_RPARA& of the same
+    // number (src/parameters.c:328-358): found like the filter family.
+    int na = 0;
+    for (auto& kv : p.order) {
+        const std::string& k = kv.first;
+        if (k.size() > 9 && k.compare(0, 9, "ANA_EVENT") == 0 && is_digit_str(k.substr(9))) {
+            if (na >= OCFD_MAX_ANA) break;
+            const std::string num = k.substr(9);
+            auto e = tokens(p.kv["ANA_EVENT" + num]);
+            for (int q = 0; q < 2 && q < (int)e.size(); q++) c->ana_event[na][q] = std::atoi(e[q].c_str());
+            auto t = tokens(p.kv["ANA_NPARA" + num]);
+            c->n_ana_npara[na] = (int)std::min<size_t>(t.size(), OCFD_ANA_NPARA);
+            for (int q = 0; q < c->n_ana_npara[na]; q++) c->ana_npara[na][q] = std::atoi(t[q].c_str());
+            auto r = tokens(p.kv["ANA_RPARA" + num]);
+            c->n_ana_rpara[na] = (int)std::min<size_t>(r.size(), OCFD_ANA_NPARA);
+            for (int q = 0; q < c->n_ana_rpara[na]; q++) c->ana_rpara[na][q] = std::atof(r[q].c_str());
+            na++;
+        }
+    }
+    c->nana = na;
     // extension key (ignored by the reference, which skips unknown keys)
     {
         auto it = p.kv.find("PRECISION");

@@ -46,6 +46,8 @@ template <typename S> void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st)
 // each, starting at component c0 (0..3).
 template <typename S> int visc_div_components();
 template <typename S> void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st);
+// Q criterion of u, v, w at every interior point into q (LAP layout, FP64).
+template <typename S> void launch_qcrit(const ViscArgsT<S>& a, double* q, hipStream_t st);
 
 // Periodic face fill (exchange_boundary_?_packed_dev at one rank): copy the
 // 4-layer faces of `nf` fields (stride fs) in direction dir.
@@ -154,6 +156,11 @@ int diag_sum_blocks(const Geo& g);
 template <typename S>
 void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, double* partT, int* scan_part,
                  hipStream_t st);
+// Running time average of the primitives: avg (5 FP64 fields, stride fs)
+// becomes (n avg + x)/(n + 1) at every interior point.
+template <typename S>
+void launch_time_avg(const Geo& g, const S* d, const S* u, const S* v, const S* w, const S* T, double* avg,
+                     long long fs, int n, hipStream_t st);
 
 // Learned sensor (k_mlp.hip): parameters W1 [32][24], b1, W2 [32][32], b2,
 // W3, b3 as 1889 floats.

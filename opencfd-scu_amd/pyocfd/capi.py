@@ -18,6 +18,8 @@ LIB_PATH = PKG_DIR / "libocfd_hip.so"
 LAP = 4
 NVARS = 5
 MAX_FILTERS = 10
+MAX_ANA = 16
+ANA_NPARA = 100
 
 CD6, CD8 = 203, 204
 UP7, WENO5, WENO7, WENO7_SYMBO, WENO7_SYMBO_LIM, NND2 = 301, 302, 303, 304, 305, 306
@@ -55,6 +57,11 @@ class OcfdConfig(C.Structure):
         ("filter_npara", (C.c_int * 11) * MAX_FILTERS),
         ("filter_rpara", (C.c_double * 3) * MAX_FILTERS),
         ("precision", C.c_int),
+        ("nana", C.c_int),
+        ("ana_event", (C.c_int * 2) * MAX_ANA),
+        ("n_ana_npara", C.c_int * MAX_ANA), ("n_ana_rpara", C.c_int * MAX_ANA),
+        ("ana_npara", (C.c_int * ANA_NPARA) * MAX_ANA),
+        ("ana_rpara", (C.c_double * ANA_NPARA) * MAX_ANA),
     ]
 
 
@@ -62,7 +69,9 @@ def make_config(**kw) -> OcfdConfig:
     """Build an ``OcfdConfig`` from keyword arguments (field names above).
 
     Array fields accept sequences; ``filters`` is a list of
-    ``(npara11, rpara3)`` pairs; ``bc_npara``/``bc_rpara`` set their counts.
+    ``(npara11, rpara3)`` pairs; ``analysis`` is a list of
+    ``(style, kstep, npara, rpara)`` tuples (ANA_EVENT& / ANA_NPARA& /
+    ANA_RPARA&); ``bc_npara``/``bc_rpara`` set their counts.
     """
     c = OcfdConfig()
     c.npx0 = c.npy0 = c.npz0 = 1
@@ -76,6 +85,7 @@ def make_config(**kw) -> OcfdConfig:
     c.kstep_save = 1 << 30
     c.init_stat = 1
     filters = kw.pop("filters", [])
+    analysis = kw.pop("analysis", [])
     for k, v in kw.items():
         if k in ("iperiodic", "jac_bound", "dif_bound", "non_ref", "hy_dp_intv"):
             arr = getattr(c, k)
@@ -99,7 +109,24 @@ def make_config(**kw) -> OcfdConfig:
             c.filter_npara[m][i] = int(npara[i])
         for i in range(3):
             c.filter_rpara[m][i] = float(rpara[i])
+    set_analysis(c, analysis)
     return c
+
+
+def set_analysis(c: OcfdConfig, events):
+    """Replace the analysis events of a config: (style, kstep, npara, rpara)."""
+    assert len(events) <= MAX_ANA
+    c.nana = len(events)
+    for m, ev in enumerate(events):
+        style, kstep = ev[0], ev[1]
+        npara = list(ev[2]) if len(ev) > 2 else []
+        rpara = list(ev[3]) if len(ev) > 3 else []
+        c.ana_event[m][0], c.ana_event[m][1] = int(style), int(kstep)
+        c.n_ana_npara[m], c.n_ana_rpara[m] = len(npara), len(rpara)
+        for i, v in enumerate(npara):
+            c.ana_npara[m][i] = int(v)
+        for i, v in enumerate(rpara):
+            c.ana_rpara[m][i] = float(v)
 
 
 def config_to_dict(c: OcfdConfig) -> dict:
@@ -174,6 +201,10 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "ocfd_diagnostics": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ocfd_get_hybrid_flags": (C.c_int, [vp, i8p, i8p, i8p]),
+        "ocfd_analysis": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ocfd_get_q": (C.c_int, [vp, dp]),
+        "ocfd_get_time_average": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "ocfd_hybrid_proportions": (C.c_int, [vp, dp]),
         "ocfd_profile_enable": (C.c_int, [vp, C.c_int]),
         "ocfd_profile_count": (C.c_int, [vp]),
         "ocfd_profile_entry": (C.c_int, [vp, C.c_int, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
@@ -402,6 +433,30 @@ class Solver:
         e, t, nn, ng = C.c_double(), C.c_double(), C.c_longlong(), C.c_longlong()
         _check(self.lib.ocfd_diagnostics(self.h, C.byref(e), C.byref(t), C.byref(nn), C.byref(ng)), "diagnostics")
         return e.value, t.value, nn.value, ng.value
+
+    def analysis(self, style, event=0):
+        """OCFD_ana: run analysis `style` with the parameters of event `event`
+        (files go to the working directory)."""
+        _check(self.lib.ocfd_analysis(self.h, int(style), int(event)), f"analysis {style}")
+
+    def get_q(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        _check(self.lib.ocfd_get_q(self.h, out), "get_q")
+        return out.reshape(self.shape)
+
+    def get_time_average(self):
+        """(d, u, v, w, T averages, samples, accumulated time)."""
+        n = self.nx * self.ny * self.nz
+        a = [np.empty(n) for _ in range(5)]
+        ns, tt = C.c_int(), C.c_double()
+        _check(self.lib.ocfd_get_time_average(self.h, *a, C.byref(ns), C.byref(tt)), "get_time_average")
+        return [x.reshape(self.shape) for x in a], ns.value, tt.value
+
+    def hybrid_proportions(self):
+        """(3, 3): fractions of flag types 1, 2, 3 for x, y, z."""
+        out = np.empty(9)
+        _check(self.lib.ocfd_hybrid_proportions(self.h, out), "hybrid_proportions")
+        return out.reshape(3, 3)
 
     def set_sensor(self, params=None, lo=0.2, hi=0.8):
         """Learned hybrid sensor with these 1889 parameters (None: plain)."""

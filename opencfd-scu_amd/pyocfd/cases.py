@@ -121,6 +121,12 @@ def config_text(c: capi.OcfdConfig) -> str:
     for m in range(c.nfilter):
         L.append(f"FILTER_NPARA{m} = " + " ".join(str(c.filter_npara[m][i]) for i in range(11)))
         L.append(f"FILTER_RPARA{m} = " + " ".join(repr(c.filter_rpara[m][i]) for i in range(3)))
+    for m in range(c.nana):
+        L.append(f"ANA_EVENT{m} = {c.ana_event[m][0]} {c.ana_event[m][1]}")
+        if c.n_ana_npara[m]:
+            L.append(f"ANA_NPARA{m} = " + " ".join(str(c.ana_npara[m][i]) for i in range(c.n_ana_npara[m])))
+        if c.n_ana_rpara[m]:
+            L.append(f"ANA_RPARA{m} = " + " ".join(repr(c.ana_rpara[m][i]) for i in range(c.n_ana_rpara[m])))
     return "\n".join(L) + "\n"
 
 

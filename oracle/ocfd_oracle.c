@@ -1266,13 +1266,14 @@ static void du_invis(orc *o) {
 
 /* du_vis_Jacobian3d (src/OCFD_Stream.cu:307-335) with du_viscous_Jacobian3d_*
  * (src/OCFD_NS_Jacobian3d.cu:58-440) and YF_Pe_XF (src/commen_kernel.cu:158-169) */
-static void du_vis(orc *o) {
+/* Derivatives along xi/eta/zeta of the first nf of u,v,w,T with the viscous
+ * scheme and its D0 closures: grad[3*f + dir] (OCFD_dx0/dy0/dz0,
+ * src/OCFD_Schemes_Choose.cu:19-84). */
+static void central_grads(orc *o, int nf) {
     int nx = o->nx, ny = o->ny, nz = o->nz;
-    long N = o->nint;
     const double *src4[4] = {o->u, o->v, o->w, o->T};
     int sv = o->c.scheme_vis;
-    /* 12 gradients: grad[3*f + dir] for f in u,v,w,T */
-    for (int fI = 0; fI < 4; fI++)
+    for (int fI = 0; fI < nf; fI++)
         for (int dir = 0; dir < 3; dir++) {
             int n = dir == 0 ? nx : dir == 1 ? ny : nz;
             long fs = dir == 0 ? 1 : dir == 1 ? o->NX2 : (long)o->NX2 * o->NY2;
@@ -1288,6 +1289,14 @@ static void du_vis(orc *o) {
                     line_central(sv, n, fl, fs, h, bl, br, o->grad[3 * fI + dir] + I0(o, X, Y, Z), os);
                 }
         }
+}
+
+static void du_vis(orc *o) {
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    long N = o->nint;
+    int sv = o->c.scheme_vis;
+    /* 12 gradients: grad[3*f + dir] for f in u,v,w,T */
+    central_grads(o, 4);
     const double *uk = o->grad[0], *ui = o->grad[1], *us = o->grad[2];
     const double *vk = o->grad[3], *vi = o->grad[4], *vs = o->grad[5];
     const double *wk = o->grad[6], *wi = o->grad[7], *ws = o->grad[8];
@@ -1553,5 +1562,34 @@ int orc_get_flags(orc *o, signed char *fx, signed char *fy, signed char *fz) {
     if (fx) for (long i = 0; i < n0; i++) fx[i] = (signed char)o->flag[0][i];
     if (fy) for (long i = 0; i < n1; i++) fy[i] = (signed char)o->flag[1][i];
     if (fz) for (long i = 0; i < n2; i++) fz[i] = (signed char)o->flag[2][i];
+    return 0;
+}
+
+/* Q criterion (get_Q / get_Q_kernal, src/OCFD_ana.cu:299-458) at the interior
+ * points from the current primitives (their ghost layers as they stand). */
+int orc_get_q(orc *o, double *q) {
+    central_grads(o, 3);
+    const double *uk = o->grad[0], *ui = o->grad[1], *us = o->grad[2];
+    const double *vk = o->grad[3], *vi = o->grad[4], *vs = o->grad[5];
+    const double *wk = o->grad[6], *wi = o->grad[7], *ws = o->grad[8];
+    for (int z = 0; z < o->nz; z++)
+        for (int y = 0; y < o->ny; y++)
+            for (int x = 0; x < o->nx; x++) {
+                long p = I0(o, x, y, z), L = IL(o, x + LAP, y + LAP, z + LAP);
+                double Akx = o->A[AKX][L], Aky = o->A[AKY][L], Akz = o->A[AKZ][L];
+                double Aix = o->A[AIX][L], Aiy = o->A[AIY][L], Aiz = o->A[AIZ][L];
+                double Asx = o->A[ASX][L], Asy = o->A[ASY][L], Asz = o->A[ASZ][L];
+                double ux = uk[p] * Akx + ui[p] * Aix + us[p] * Asx;
+                double vx = vk[p] * Akx + vi[p] * Aix + vs[p] * Asx;
+                double wx = wk[p] * Akx + wi[p] * Aix + ws[p] * Asx;
+                double uy = uk[p] * Aky + ui[p] * Aiy + us[p] * Asy;
+                double vy = vk[p] * Aky + vi[p] * Aiy + vs[p] * Asy;
+                double wy = wk[p] * Aky + wi[p] * Aiy + ws[p] * Asy;
+                double uz = uk[p] * Akz + ui[p] * Aiz + us[p] * Asz;
+                double vz = vk[p] * Akz + vi[p] * Aiz + vs[p] * Asz;
+                double wz = wk[p] * Akz + wi[p] * Aiz + ws[p] * Asz;
+                double aj = o->A[AJAC][L];
+                q[p] = (ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy) * aj * aj;
+            }
     return 0;
 }

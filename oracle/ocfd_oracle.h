@@ -46,6 +46,8 @@ int orc_get_pressure_lap(orc *o, double *p);
 int orc_get_amu(orc *o, double *amu);
 int orc_get_metrics(orc *o, double *a13);
 int orc_get_flags(orc *o, signed char *fx, signed char *fy, signed char *fz);
+/* Q criterion at the interior points (nz*ny*nx). */
+int orc_get_q(orc *o, double *q);
 
 /* Unit-level entry points used by the pinning tests. */
 /* Half-node flux of one scheme on a stencil s[0..7] laid out as the

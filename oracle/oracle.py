@@ -59,6 +59,7 @@ def lib() -> C.CDLL:
             "orc_get_amu": (C.c_int, [vp, dp]),
             "orc_get_metrics": (C.c_int, [vp, dp]),
             "orc_get_flags": (C.c_int, [vp, i8, i8, i8]),
+            "orc_get_q": (C.c_int, [vp, dp]),
             "orc_face_flux": (C.c_double, [C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_double]),
             "orc_split_point": (None, [C.c_double] * 10 + [dp]),
             "orc_line_flux_derivative": (None, [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.c_int,
@@ -173,6 +174,11 @@ class Oracle:
         fz = np.empty(nx * ny * (nz + 1), np.int8)
         self.L.orc_get_flags(self.h, fx, fy, fz)
         return fx.reshape(nz, ny, nx + 1), fy.reshape(nz, ny + 1, nx), fz.reshape(nz + 1, ny, nx)
+
+    def get_q(self):
+        out = np.empty(self.nx * self.ny * self.nz)
+        self.L.orc_get_q(self.h, out)
+        return out.reshape(self.shape)
 
 
 # ---- unit-level helpers -------------------------------------------------

@@ -30,6 +30,7 @@ def test_every_declared_symbol_is_exported(hiplib):
 
 def test_config_roundtrip_through_parser(tmp_path):
     for case in (cases.tgv(16, nsteps=3), cases.ramp(64, 32, 8, nsteps=3)):
+        capi.set_analysis(case.cfg, [(100, 5), (104, 10, [2, 1, 3, 7], [0.5]), (107, 1)])
         p = tmp_path / "opencfd-scu.in"
         p.write_text(case.input_text())
         c = capi.read_config(p)
@@ -37,6 +38,8 @@ def test_config_roundtrip_through_parser(tmp_path):
         for k in a:
             if k in ("filter_npara", "filter_rpara"):
                 assert a[k][: case.cfg.nfilter] == b[k][: case.cfg.nfilter], k
+            elif k in ("ana_event", "n_ana_npara", "n_ana_rpara", "ana_npara", "ana_rpara"):
+                assert a[k][: case.cfg.nana] == b[k][: case.cfg.nana], k
             elif k in ("bc_npara", "bc_rpara"):
                 n = case.cfg.n_bc_npara if k == "bc_npara" else case.cfg.n_bc_rpara
                 assert a[k][:n] == b[k][:n], k
@@ -78,6 +81,31 @@ def test_parser_follows_reference_key_rules(tmp_path):
     assert list(c.filter_rpara[0]) == [0.5, 1e-5, 100.0]
     assert c.n_bc_npara == 4 and c.n_bc_rpara == 8 and c.bc_rpara[7] == 14.0
     assert c.re == 0.0
+
+
+def test_parser_reads_analysis_events(tmp_path):
+    """ANA_EVENT& = style Kstep with ANA_NPARA& / ANA_RPARA& of the same
+    number, in file order (src/parameters.c:328-358)."""
+    text = "\n".join([
+        "GRID_3D = 8 8 8",
+        "SCHEME_INVIS = WENO7",
+        "SCHEME_VIS = CD6",
+        "ANA_EVENT7 = 104 20",
+        "ANA_NPARA7 = 2 1 3 5",
+        "ANA_RPARA7 = 0.25",
+        "ANA_EVENT2 = 100 1",
+        "#ANA_EVENT3 = 101 1",
+        "ANA_EVENT = 101 1",              # no number: not an event here
+        "",
+    ])
+    p = tmp_path / "opencfd-scu.in"
+    p.write_text(text)
+    c = capi.read_config(p)
+    assert c.nana == 2
+    assert list(c.ana_event[0]) == [104, 20] and list(c.ana_event[1]) == [100, 1]
+    assert c.n_ana_npara[0] == 4 and list(c.ana_npara[0])[:4] == [2, 1, 3, 5]
+    assert c.n_ana_rpara[0] == 1 and c.ana_rpara[0][0] == 0.25
+    assert c.n_ana_npara[1] == 0 and c.n_ana_rpara[1] == 0
 
 
 def test_parser_rejects_bad_scheme(tmp_path):

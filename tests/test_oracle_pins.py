@@ -263,6 +263,39 @@ def test_metrics_of_affine_mesh():
     assert np.allclose(m[12], 1.0 / np.linalg.det(B), rtol=1e-10)
 
 
+@pytest.mark.parametrize("svis", [capi.CD6, capi.CD8])
+def test_q_criterion_of_linear_velocity(svis):
+    """Q of u = G x on an affine mesh is the sum of the principal 2x2 minors
+    of G (the second invariant of the velocity gradient), exactly up to
+    rounding: the derivatives and their one-sided closures are exact on linear
+    data and the cofactors times Ajac invert the mesh map."""
+    nx, ny, nz = 11, 10, 9
+    cfg = capi.make_config(nx=nx, ny=ny, nz=nz, iperiodic=(0, 0, 0), jac_bound=(1, 1, 1),
+                           dif_bound=(1, 1, 1, 1, 1, 1), re=100.0, ama=0.5, dt=1e-3, scheme_vis=svis)
+    rng = np.random.default_rng(5)
+    B = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    G = rng.standard_normal((3, 3))
+    hx, hy, hz = 1 / (nx - 1), 1 / (ny - 1), 1 / (nz - 1)
+    k, j, i = np.meshgrid(np.arange(nz) * hz, np.arange(ny) * hy, np.arange(nx) * hx, indexing="ij")
+    X = [B[r, 0] * i + B[r, 1] * j + B[r, 2] * k for r in range(3)]
+    vel = [G[r, 0] * X[0] + G[r, 1] * X[1] + G[r, 2] * X[2] for r in range(3)]
+    o = O.Oracle(cfg)
+    o.load_mesh(*X)
+    one = np.ones((nz, ny, nx))
+    o.load_flow(one, *vel, one)
+    q = o.get_q()
+    want = (G[0, 0] * G[1, 1] - G[0, 1] * G[1, 0] + G[0, 0] * G[2, 2] - G[0, 2] * G[2, 0]
+            + G[1, 1] * G[2, 2] - G[1, 2] * G[2, 1])
+    assert np.abs(q - want).max() <= 1e-10 * max(1.0, abs(want))
+    # a rotation about z has Q = w^2 > 0, a pure strain Q < 0
+    vel = [-X[1], X[0], 0 * X[2]]
+    o.load_flow(one, *vel, one)
+    assert np.allclose(o.get_q(), 1.0, rtol=0, atol=1e-10)
+    vel = [X[0], -X[1], 0 * X[2]]
+    o.load_flow(one, *vel, one)
+    assert np.allclose(o.get_q(), -1.0, rtol=0, atol=1e-10)
+
+
 def test_sutherland_at_reference_temperature():
     c = cases.tgv(8, nsteps=1)
     n = 8
