@@ -59,7 +59,7 @@ typedef struct ocfd_config {
     int nx, ny, nz;              /* GRID_3D                                  */
     int npx0, npy0, npz0;        /* PARALLEL_3D (must be 1 1 1 here)         */
     int stream_mode;             /* STREAM (0 supported)                     */
-    int charteric;               /* CHARTERIC (0 supported)                  */
+    int charteric;               /* CHARTERIC (1: SYMBO / hybrid_auto only)  */
     int test;                    /* TEST                                     */
     int iperiodic[3];            /* IPERIODIC                                */
     int jac_bound[3];            /* JAC_BOUND                                */

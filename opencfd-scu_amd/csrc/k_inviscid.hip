@@ -141,10 +141,8 @@ __device__ __forceinline__ void face_pm(const R* gp, const R* gm, int c, int r, 
 // v-5 at stencil node k, k = 0..7 for nodes j-3..j+4).  Same arithmetic as
 // face_pm, with the per-variable branching hoisted.
 template <typename R, typename W, int SCH, typename LD>
-__device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, int hy, R tvlim, R tvmax, R* hp,
-                                         R* hm) {
+__device__ __forceinline__ void face_all_p(LD&& ld, int c, int r, int bl, int br, int hy, R tvlim, R tvmax, R* hp) {
     constexpr int kaP = Ext<SCH, 0>::ka, kbP = Ext<SCH, 0>::kb;
-    constexpr int kaM = Ext<SCH, 1>::ka, kbM = Ext<SCH, 1>::kb;
     // P: 0..3 left closures (c), 4..6 right closures, 7 scheme, 8 WENO7, 9 WENO5
     int pc;
     if (bl && c <= -kaP) pc = c;
@@ -196,9 +194,14 @@ __device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, 
             else hp[m] = weno5_up_rightplus<R, W>(g + 1);
         }
     }
+}
+// M half-node values (ld(5+m, k) = f- of variable m at node j-3+k).
+template <typename R, typename W, int SCH, typename LD>
+__device__ __forceinline__ void face_all_m(LD&& ld, int c, int r, int bl, int br, int hy, R* hm) {
+    constexpr int kaM = Ext<SCH, 1>::ka, kbM = Ext<SCH, 1>::kb;
     // M: 0..2 left closures (c), 3..6 right closures, 7 scheme, 8 WENO7, 9 WENO5
     int mc;
-    if (bl && c < -kaM) mc = c;
+    if (bl && c < -kaM) mc = c < 2 ? c : 2;  // c >= 2: plain WENO5
     else if (br && r >= -kbM - 1) mc = r == -1 ? 3 : r == -2 ? 4 : r == -3 ? 5 : 6;
     else if (SCH == OCFD_HYBRIDAUTO) mc = hy == 1 ? 7 : hy == 2 ? 8 : 9;
     else mc = 7;
@@ -246,6 +249,12 @@ __device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, 
             else hm[m] = weno5_dn_rightplus<R, W>(g + 2);
         }
     }
+}
+template <typename R, typename W, int SCH, typename LD>
+__device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, int hy, R tvlim, R tvmax, R* hp,
+                                         R* hm) {
+    face_all_p<R, W, SCH>(ld, c, r, bl, br, hy, tvlim, tvmax, hp);
+    face_all_m<R, W, SCH>(ld, c, r, bl, br, hy, hm);
 }
 
 // x sweep: a block owns 4 rows of 56 output nodes; the f+/f- of the 64 nodes
@@ -552,6 +561,249 @@ __global__ void __launch_bounds__(256) k_march(SweepArgsT<S> A) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Characteristic-space reconstruction (CHARTERIC = 1):
+// OCFD_weno7_SYMBO_character_P/M_kernel and
+// OCFD_HybridAuto_character_P/M_Jameson_kernel (src/OCFD_flux_charteric.cu:
+// 754-908, 1000-1268).  Same three phases as the physical sweeps, on one
+// tile shape for all directions; per face j+1/2 in phase 2:
+//   * the face state (means of u, v, w, c and the cofactor row over nodes j
+//     and j+1) gives the left and right eigenvector matrices L and R
+//     (get_para_charteric_*, flux_charteric_ptoc/ctop_kernel, :321-752);
+//   * the eight f+ vectors of the stencil are projected with L, the five
+//     characteristic variables reconstructed with the closures of the
+//     physical path, and R maps the result back; the same for f-.  A hybrid
+//     face with flag 1 (UP7) stays in physical space.
+// The face state, L and R are FP64 on both paths.  Phase 3 follows
+// put_du_character_p/m_kernel (:16-119): with a closure on one face only, the
+// boundary node of that face takes a physical-space difference of the split
+// flux (OCFD_bound_character_kernel_p/m, :121-319) instead of the face
+// difference, the non-reflecting zero applies only there, and the low-side
+// M difference is not taken in y (the reference's test compares the y flag
+// with the z value).
+// ---------------------------------------------------------------------------
+struct CharFace {
+    double L[25], R[25];
+};
+__device__ __forceinline__ void char_face(double u0, double v0, double w0, double c0, double a10, double a20,
+                                          double a30, double u1, double v1, double w1, double c1, double a11,
+                                          double a21, double a31, double gamma, CharFace& F) {
+    const double u = (u0 + u1) * 0.5, v = (v0 + v1) * 0.5, w = (w0 + w1) * 0.5, c = (c0 + c1) * 0.5;
+    const double a1 = (a10 + a11) * 0.5, a2 = (a20 + a21) * 0.5, a3 = (a30 + a31) * 0.5;
+    double ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+    const double n1 = a1 / ss, n2 = a2 / ss, n3 = a3 / ss;
+    double l1, l2, l3;
+    if (fabs(n3) <= fabs(n2)) {
+        ss = sqrt(n1 * n1 + n2 * n2);
+        l1 = -n2 / ss; l2 = n1 / ss; l3 = 0.0;
+    } else {
+        ss = sqrt(n1 * n1 + n3 * n3);
+        l1 = -n3 / ss; l2 = 0.0; l3 = n1 / ss;
+    }
+    const double m1 = n2 * l3 - n3 * l2, m2 = n3 * l1 - n1 * l3, m3 = n1 * l2 - n2 * l1;
+    const double KK = (gamma - 1.0) / (c * c);
+    const double X1 = 1.0 / (2.0 * c);
+    const double un = u * n1 + v * n2 + w * n3;
+    const double v2 = (u * u + v * v + w * w) * 0.5;
+    double* L = F.L;
+    L[0] = 1.0 - KK * v2; L[1] = KK * u; L[2] = KK * v; L[3] = KK * w; L[4] = -KK;
+    L[5] = -(u * l1 + v * l2 + w * l3); L[6] = l1; L[7] = l2; L[8] = l3; L[9] = 0.0;
+    L[10] = -(u * m1 + v * m2 + w * m3); L[11] = m1; L[12] = m2; L[13] = m3; L[14] = 0.0;
+    L[15] = KK * v2 * 0.5 + X1 * un;
+    L[16] = -X1 * n1 - KK * u * 0.5; L[17] = -X1 * n2 - KK * v * 0.5; L[18] = -X1 * n3 - KK * w * 0.5;
+    L[19] = KK * 0.5;
+    L[20] = L[15] - 2 * X1 * un;
+    L[21] = 2 * X1 * n1 + L[16]; L[22] = 2 * X1 * n2 + L[17]; L[23] = 2 * X1 * n3 + L[18]; L[24] = L[19];
+    const double H = v2 + 1.0 / KK;
+    double* R = F.R;
+    R[0] = 1.0; R[1] = 0.0; R[2] = 0.0; R[3] = 1.0; R[4] = 1.0;
+    R[5] = u; R[6] = l1; R[7] = m1; R[8] = u - c * n1; R[9] = 2 * u - R[8];
+    R[10] = v; R[11] = l2; R[12] = m2; R[13] = v - c * n2; R[14] = 2 * v - R[13];
+    R[15] = w; R[16] = l3; R[17] = m3; R[18] = w - c * n3; R[19] = 2 * w - R[18];
+    R[20] = v2; R[21] = u * l1 + v * l2 + w * l3; R[22] = u * m1 + v * m2 + w * m3;
+    R[23] = H - c * un; R[24] = 2 * H - R[23];
+}
+__device__ __forceinline__ void mv5(const double* M, const double* x, double* y) {
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        y[i] = M[5 * i] * x[0] + M[5 * i + 1] * x[1] + M[5 * i + 2] * x[2] + M[5 * i + 3] * x[3] + M[5 * i + 4] * x[4];
+}
+
+template <int DIR> struct CharTile { static constexpr int NS = 24, NA = 8; };
+template <> struct CharTile<0> { static constexpr int NS = 56, NA = 4; };
+
+template <int DIR, int SCH, bool FIRST, typename S>
+__global__ void __launch_bounds__(256) k_char(SweepArgsT<S> A) {
+    constexpr int NS = CharTile<DIR>::NS, NA = CharTile<DIR>::NA, NN = NS + 8, NF = NS + 1;
+    static_assert(NN * NA == 256 && NF * NA <= 256, "one node and at most one face per thread");
+    __shared__ S sh[10 * NN * NA];
+    __shared__ S shb[2][10][NA];  // boundary differences: [low/high][P 0..4, M 5..9][column]
+    const Geo& g = A.g;
+    const int tid = threadIdx.x;
+    const Blk B = xcd_block();
+    const int s0 = (DIR == 0 ? B.x : B.y) * NS, a0 = (DIR == 0 ? B.y : B.x) * NA, k2 = B.z;
+    const int n_dir = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int n_acr = DIR == 0 ? g.ny : g.nx;
+    auto lin = [&](int s, int a) -> long long {
+        return DIR == 0 ? gidx(g, s, a, k2) : DIR == 1 ? gidx(g, a, s, k2) : gidx(g, a, k2, s);
+    };
+    // lanes run along x: along the sweep for x, across it for y and z
+    auto at = [](int v, int n, int a) { return DIR == 0 ? (v * NA + a) * NN + n : (v * NN + n) * NA + a; };
+    auto fat = [](int v, int f, int a) { return DIR == 0 ? (v * NA + a) * NF + f : (v * NF + f) * NA + a; };
+    const int fyc = A.bl && A.br ? 3 : A.bl ? 1 : A.br ? 2 : 0;  // closures: both, low, high, none
+
+    // ---- phase 1: split into LDS
+    {
+        const int n = DIR == 0 ? tid % NN : tid / NA, a = DIR == 0 ? tid / NN : tid % NA;
+        const int s = s0 - 4 + n, ac = a0 + a;
+        S fp[5], fm[5];
+        if (s <= n_dir + 3 && ac < n_acr) {
+            const long long q = lin(s, ac);
+            const S d = A.d[q], u = A.u[q], v = A.v[q], w = A.w[q], T = A.T[q];
+            const S cc = sqrt(T) * S(A.c.rama);
+            sw_split<S>(d * S(A.c.tmp0), u, v, w, cc, S(A.m1[q]), S(A.m2[q]), S(A.m3[q]), A.c, fp, fm);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; m++) { fp[m] = S(0); fm[m] = S(0); }
+        }
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            sh[at(m, n, a)] = fp[m];
+            sh[at(5 + m, n, a)] = fm[m];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: one face per thread
+    S hp[5], hm[5];
+    const bool fthread = tid < NF * NA;
+    const int f_loc = DIR == 0 ? tid % NF : tid / NA, a_loc = DIR == 0 ? tid / NF : tid % NA;
+    const int j = s0 - 1 + f_loc, ac = a0 + a_loc;
+    const bool fvalid = fthread && j <= n_dir - 1 && ac < n_acr;
+    if (fvalid) {
+        auto ld = [&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; };
+        int hy = 0;
+        if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[lin(j, ac)];
+        const int c = j + 1, r = j - n_dir;
+        if (SCH == OCFD_HYBRIDAUTO && hy == 1) {
+            face_all<S, double, SCH>(ld, c, r, A.bl, A.br, hy, S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
+        } else {
+            const long long q0 = lin(j, ac), q1 = lin(j + 1, ac);
+            const double rama = A.c.rama;
+            CharFace F;
+            char_face(double(A.u[q0]), double(A.v[q0]), double(A.w[q0]), sqrt(double(A.T[q0])) * rama, A.m1[q0],
+                      A.m2[q0], A.m3[q0], double(A.u[q1]), double(A.v[q1]), double(A.w[q1]),
+                      sqrt(double(A.T[q1])) * rama, A.m1[q1], A.m2[q1], A.m3[q1], A.c.gamma, F);
+            S cs[5][8];
+            double x[5], y[5], hc[5];
+            S hcs[5];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) x[m] = double(ld(m, k));
+                mv5(F.L, x, y);
+#pragma unroll
+                for (int m = 0; m < 5; m++) cs[m][k] = S(y[m]);
+            }
+            face_all_p<S, double, SCH>([&](int v, int k) { return cs[v][k]; }, c, r, A.bl, A.br, hy,
+                                       S(A.c.tv_lim), S(A.c.tv_max), hcs);
+#pragma unroll
+            for (int m = 0; m < 5; m++) hc[m] = double(hcs[m]);
+            mv5(F.R, hc, y);
+#pragma unroll
+            for (int m = 0; m < 5; m++) hp[m] = S(y[m]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+#pragma unroll
+                for (int m = 0; m < 5; m++) x[m] = double(ld(5 + m, k));
+                mv5(F.L, x, y);
+#pragma unroll
+                for (int m = 0; m < 5; m++) cs[m][k] = S(y[m]);
+            }
+            face_all_m<S, double, SCH>([&](int v, int k) { return cs[v - 5][k]; }, c, r, A.bl, A.br, hy, hcs);
+#pragma unroll
+            for (int m = 0; m < 5; m++) hc[m] = double(hcs[m]);
+            mv5(F.R, hc, y);
+#pragma unroll
+            for (int m = 0; m < 5; m++) hm[m] = S(y[m]);
+        }
+        // physical-space differences of the boundary nodes (node j-3+k at k)
+        if (j == 0 && fyc == 1) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                shb[0][m][a_loc] = ld(m, 4) - ld(m, 3);
+                const S f0 = ld(5 + m, 3), f1 = ld(5 + m, 4), f2 = ld(5 + m, 5);
+                shb[0][5 + m][a_loc] =
+                    (f1 - S(0.5) * d_minmod2(f2 - f1, f1 - f0)) - (f0 - S(0.5) * d_minmod2(f1 - f0, f1 - f0));
+            }
+        }
+        if (j == n_dir - 1 && fyc == 2) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const S a = ld(m, 3), b = ld(m, 2), cc = ld(m, 1);
+                shb[1][m][a_loc] = (a + S(0.5) * d_minmod2(a - b, a - b)) - (b + S(0.5) * d_minmod2(a - b, b - cc));
+                shb[1][5 + m][a_loc] = ld(5 + m, 3) - ld(5 + m, 2);
+            }
+        }
+    }
+    __syncthreads();
+    if (fthread) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            sh[fat(m, f_loc, a_loc)] = fvalid ? hp[m] : S(0);
+            sh[fat(5 + m, f_loc, a_loc)] = fvalid ? hm[m] : S(0);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: du at nodes, P then M
+    if (tid < NS * NA) {
+        const int n = DIR == 0 ? tid % NS : tid / NA, a = DIR == 0 ? tid / NS : tid % NA;
+        const int i = s0 + n, ai = a0 + a;
+        if (i < n_dir && ai < n_acr) {
+            const long long q = lin(i, ai);
+            const double ajac = A.jac[q];
+            const double rh = 1.0 / A.h;
+            const bool lowP = fyc == 1 && i == 0, highP = fyc == 2 && i == n_dir - 1;
+            const bool highM = fyc == 2 && i == n_dir - 1, lowM = fyc == 1 && i == 0 && DIR != 1;
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                double tP, tM;
+                if (lowP) tP = A.nrp ? 0.0 : -ajac * double(shb[0][m][a]) * rh;
+                else if (highP) tP = -ajac * double(shb[1][m][a]) * rh;
+                else tP = -ajac * double(sh[fat(m, n + 1, a)] - sh[fat(m, n, a)]) * rh;
+                if (highM) tM = A.nrm ? 0.0 : -ajac * double(shb[1][5 + m][a]) * rh;
+                else if (lowM) tM = -ajac * double(shb[0][5 + m][a]) * rh;
+                else tM = -ajac * double(sh[fat(5 + m, n + 1, a)] - sh[fat(5 + m, n, a)]) * rh;
+                S* dv = A.du + (long long)m * A.fs;
+                double acc = FIRST ? 0.0 : double(dv[q]);
+                acc = acc + tP;
+                acc = acc + tM;
+                dv[q] = S(acc);
+            }
+        }
+    }
+}
+
+template <int DIR, int SCH, typename S>
+void launch_char_dir(const SweepArgsT<S>& a, bool first, hipStream_t st) {
+    constexpr int NS = CharTile<DIR>::NS, NA = CharTile<DIR>::NA;
+    const Geo& g = a.g;
+    const int n_dir = DIR == 0 ? g.nx : DIR == 1 ? g.ny : g.nz;
+    const int n_acr = DIR == 0 ? g.ny : g.nx;
+    const int third = DIR == 0 ? g.nz : DIR == 1 ? g.nz : g.ny;
+    const dim3 grid = DIR == 0 ? dim3((n_dir + NS - 1) / NS, (n_acr + NA - 1) / NA, third)
+                               : dim3((n_acr + NA - 1) / NA, (n_dir + NS - 1) / NS, third);
+    if (first) hipLaunchKernelGGL((k_char<DIR, SCH, true, S>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_char<DIR, SCH, false, S>), grid, dim3(256), 0, st, a);
+}
+template <int SCH, typename S>
+void launch_char_scheme(int dir, const SweepArgsT<S>& a, bool first, hipStream_t st) {
+    if (dir == 0) launch_char_dir<0, SCH>(a, first, st);
+    else if (dir == 1) launch_char_dir<1, SCH>(a, first, st);
+    else launch_char_dir<2, SCH>(a, first, st);
+}
+
 #ifndef OCFD_LMARCH
 #define OCFD_LMARCH 32  // output nodes per marching chunk (k_march)
 #endif
@@ -624,3 +876,16 @@ int launch_sweep(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStr
 }
 template int launch_sweep<double>(int, int, const SweepArgsT<double>&, bool, hipStream_t);
 template int launch_sweep<float>(int, int, const SweepArgsT<float>&, bool, hipStream_t);
+
+template <typename S>
+int launch_sweep_char(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStream_t st) {
+    switch (scheme) {
+    case OCFD_WENO7_SYMBO: launch_char_scheme<OCFD_WENO7_SYMBO>(dir, a, first, st); break;
+    case OCFD_WENO7_SYMBO_LIM: launch_char_scheme<OCFD_WENO7_SYMBO_LIM>(dir, a, first, st); break;
+    case OCFD_HYBRIDAUTO: launch_char_scheme<OCFD_HYBRIDAUTO>(dir, a, first, st); break;
+    default: return -1;
+    }
+    return 0;
+}
+template int launch_sweep_char<double>(int, int, const SweepArgsT<double>&, bool, hipStream_t);
+template int launch_sweep_char<float>(int, int, const SweepArgsT<float>&, bool, hipStream_t);

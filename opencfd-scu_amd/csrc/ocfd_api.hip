@@ -367,7 +367,10 @@ int rhs(ocfd_ctx* x, int krk) {
         a.bl = C.dif_bound[2 * dir]; a.br = C.dif_bound[2 * dir + 1];
         a.nrp = C.non_ref[2 * dir]; a.nrm = C.non_ref[2 * dir + 1];
         int lrc = 0;
-        TRY(timed(x, names[dir], [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
+        if (C.charteric)
+            TRY(timed(x, names[dir], [&] { lrc = launch_sweep_char(C.scheme_invis, dir, a, dir == 0, x->st); }));
+        else
+            TRY(timed(x, names[dir], [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
         if (lrc) return fail(OCFD_ERR_UNSUPPORTED, "scheme not supported");
     }
     ViscArgsT<S> va;
@@ -575,7 +578,13 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     if ((C.npx0 > 1) || (C.npy0 > 1) || (C.npz0 > 1))
         return fail(OCFD_ERR_UNSUPPORTED, "PARALLEL_3D must be 1 1 1 (one GPU)");
     if (C.stream_mode != 0) return fail(OCFD_ERR_UNSUPPORTED, "STREAM=1 needs multiple ranks; use STREAM=0");
-    if (C.charteric != 0) return fail(OCFD_ERR_UNSUPPORTED, "CHARTERIC=1 is not supported");
+    // src/OCFD_Schemes_Choose.cu:103-145: the characteristic kernels exist for
+    // WENO7_SYMBO(_LIM) and hybrid_auto only (the reference prints a message
+    // and leaves the inviscid term out for the others)
+    if (C.charteric != 0 && C.charteric != 1) return fail(OCFD_ERR_CONFIG, "CHARTERIC must be 0 or 1");
+    if (C.charteric == 1 && C.scheme_invis != OCFD_WENO7_SYMBO && C.scheme_invis != OCFD_WENO7_SYMBO_LIM &&
+        C.scheme_invis != OCFD_HYBRIDAUTO)
+        return fail(OCFD_ERR_UNSUPPORTED, "CHARTERIC=1 needs WENO7_SYMBO, WENO7_SYMBO_LIM or SCHEME_HYBRIDAUTO");
     if (!supported_scheme(C.scheme_invis)) return fail(OCFD_ERR_CONFIG, "SCHEME_INVIS not recognised");
     if (C.scheme_vis != OCFD_CD6 && C.scheme_vis != OCFD_CD8) return fail(OCFD_ERR_CONFIG, "SCHEME_VIS not recognised");
     if (C.scheme_invis == OCFD_HYBRIDAUTO && C.hy_style != 2)

@@ -26,6 +26,11 @@ struct SweepArgsT {
 };
 template <typename S>
 int launch_sweep(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStream_t st);
+// The same sweep with the reconstruction in characteristic space
+// (CHARTERIC = 1): WENO7_SYMBO, WENO7_SYMBO_LIM and SCHEME_HYBRIDAUTO only
+// (returns -1 for the other schemes, which the reference cannot run so).
+template <typename S>
+int launch_sweep_char(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStream_t st);
 
 // Viscous terms.
 template <typename S>

@@ -611,6 +611,154 @@ static void line_flux(int scheme, int side, int n, const double *f, long fs,
     free(hf);
 }
 
+/* ---- characteristic-space reconstruction (CHARTERIC = 1) ------------- */
+
+/* Face state of the projection, get_para_charteric_p/m_kernel
+ * (src/OCFD_flux_charteric.cu:321-520): means of u, v, w, c and the cofactor
+ * row over the face's two nodes; n = A/|A|; l a unit vector normal to n
+ * (no z component when |n3| <= |n2|, else no y component); m = n x l;
+ * KK = (gamma-1)/c^2.  s0, s1 = u v w c a1 a2 a3 of the two nodes.
+ * P = u v w c n1 n2 n3 l1 l2 l3 m1 m2 m3 KK. */
+static void char_para(const double *s0, const double *s1, double gamma, double *P) {
+    double u1 = (s0[0] + s1[0]) * 0.5, v1 = (s0[1] + s1[1]) * 0.5, w1 = (s0[2] + s1[2]) * 0.5;
+    double c1 = (s0[3] + s1[3]) * 0.5;
+    double a1 = (s0[4] + s1[4]) * 0.5, a2 = (s0[5] + s1[5]) * 0.5, a3 = (s0[6] + s1[6]) * 0.5;
+    double ss = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+    double n1 = a1 / ss, n2 = a2 / ss, n3 = a3 / ss, l1, l2, l3;
+    if (fabs(n3) <= fabs(n2)) {
+        ss = sqrt(n1 * n1 + n2 * n2);
+        l1 = -n2 / ss; l2 = n1 / ss; l3 = 0.0;
+    } else {
+        ss = sqrt(n1 * n1 + n3 * n3);
+        l1 = -n3 / ss; l2 = 0.0; l3 = n1 / ss;
+    }
+    P[0] = u1; P[1] = v1; P[2] = w1; P[3] = c1;
+    P[4] = n1; P[5] = n2; P[6] = n3; P[7] = l1; P[8] = l2; P[9] = l3;
+    P[10] = n2 * l3 - n3 * l2; P[11] = n3 * l1 - n1 * l3; P[12] = n1 * l2 - n2 * l1;
+    P[13] = (gamma - 1.0) / (c1 * c1);
+}
+
+/* Left (L, flux_charteric_ptoc_kernel, :617-685) and right (R,
+ * flux_charteric_ctop_kernel, :687-752) eigenvector matrices, row-major. */
+static void char_matrices(const double *P, double *L, double *R) {
+    double u1 = P[0], v1 = P[1], w1 = P[2], c1 = P[3], n1 = P[4], n2 = P[5], n3 = P[6];
+    double l1 = P[7], l2 = P[8], l3 = P[9], m1 = P[10], m2 = P[11], m3 = P[12], KK = P[13];
+    double X1 = 1.0 / (2.0 * c1);
+    double un = u1 * n1 + v1 * n2 + w1 * n3;
+    double v2 = (u1 * u1 + v1 * v1 + w1 * w1) * 0.5;
+    L[0] = 1.0 - KK * v2; L[1] = KK * u1; L[2] = KK * v1; L[3] = KK * w1; L[4] = -KK;
+    L[5] = -(u1 * l1 + v1 * l2 + w1 * l3); L[6] = l1; L[7] = l2; L[8] = l3; L[9] = 0.0;
+    L[10] = -(u1 * m1 + v1 * m2 + w1 * m3);
+    L[11] = n2 * l3 - n3 * l2; L[12] = n3 * l1 - n1 * l3; L[13] = n1 * l2 - n2 * l1; L[14] = 0.0;
+    L[15] = KK * v2 * 0.5 + X1 * un;
+    L[16] = -X1 * n1 - KK * u1 * 0.5; L[17] = -X1 * n2 - KK * v1 * 0.5; L[18] = -X1 * n3 - KK * w1 * 0.5;
+    L[19] = KK * 0.5;
+    L[20] = L[15] - 2 * X1 * un;
+    L[21] = 2 * X1 * n1 + L[16]; L[22] = 2 * X1 * n2 + L[17]; L[23] = 2 * X1 * n3 + L[18]; L[24] = L[19];
+    double H = v2 + 1.0 / KK;
+    R[0] = 1.0; R[1] = 0.0; R[2] = 0.0; R[3] = 1.0; R[4] = 1.0;
+    R[5] = u1; R[6] = l1; R[7] = n2 * l3 - n3 * l2; R[8] = u1 - c1 * n1; R[9] = 2 * u1 - R[8];
+    R[10] = v1; R[11] = l2; R[12] = n3 * l1 - n1 * l3; R[13] = v1 - c1 * n2; R[14] = 2 * v1 - R[13];
+    R[15] = w1; R[16] = l3; R[17] = n1 * l2 - n2 * l1; R[18] = w1 - c1 * n3; R[19] = 2 * w1 - R[18];
+    R[20] = v2; R[21] = u1 * l1 + v1 * l2 + w1 * l3; R[22] = u1 * m1 + v1 * m2 + w1 * m3;
+    R[23] = H - c1 * (u1 * n1 + v1 * n2 + w1 * n3); R[24] = 2 * H - R[23];
+}
+
+static void matvec5(const double *M, const double *x, double *y) {
+    for (int i = 0; i < 5; i++)
+        y[i] = M[5 * i] * x[0] + M[5 * i + 1] * x[1] + M[5 * i + 2] * x[2] + M[5 * i + 3] * x[3] + M[5 * i + 4] * x[4];
+}
+
+/* Flux derivative of all five variables along one line in characteristic
+ * space: OCFD_weno7_SYMBO_character_P/M_kernel and
+ * OCFD_HybridAuto_character_P/M_Jameson_kernel (src/OCFD_flux_charteric.cu:
+ * 754-908, 1000-1268).  Per face j+1/2 the eight split-flux vectors of the
+ * stencil are projected with L of the face, each characteristic variable is
+ * reconstructed (with the closures of the physical path), and R maps back;
+ * a hybrid face with flag 1 stays in physical space (UP7).  du: the
+ * reference's put_du_character_p/m_kernel (:16-119): with a closure on one
+ * face only, the first (P, low closure) or last (high closure) node takes a
+ * physical-space difference of the split flux (OCFD_bound_character_kernel_p
+ * /m, :121-319) instead of the face difference, the non-reflecting zero
+ * applies only then, and the low-side M special is never taken in y (its
+ * test compares the y flag with the z value).  F[m][(q+LAP)*fs] = split flux
+ * of variable m at node q; prim[0..6] = u v w c a1 a2 a3 at the same place. */
+static void line_flux_char(int scheme, int side, int dir, int n, const double *const *F, long fs,
+                           const double *const *prim, const double *ajac, double h, int bl, int br, int nonref,
+                           const int *flags, long gs, double gamma, double *const *du, long ds, double tvlim,
+                           double tvmax) {
+    int ka1, kb1;
+    scheme_extent(scheme, side, &ka1, &kb1);
+    double *hf = (double *)malloc(sizeof(double) * 5 * (n + 1));
+    for (int j = -1; j <= n - 1; j++) {
+        double s[5][8], x[5], y[5], L[25], R[25], P[14];
+        int center = (side == 0) ? j : j + 1;
+        for (int m = 0; m < 5; m++)
+            for (int k = 0; k <= kb1 - ka1; k++) s[m][k] = F[m][(long)(center + ka1 + k + LAP) * fs];
+        int hy = flags ? flags[(long)(j + 1) * gs] : 0;
+        int proj = !(scheme == OCFD_HYBRIDAUTO && hy == 1);
+        if (proj) {
+            double s0[7], s1[7];
+            for (int t = 0; t < 7; t++) {
+                s0[t] = prim[t][(long)(j + LAP) * fs];
+                s1[t] = prim[t][(long)(j + 1 + LAP) * fs];
+            }
+            char_para(s0, s1, gamma, P);
+            char_matrices(P, L, R);
+            for (int k = 0; k <= kb1 - ka1; k++) {
+                for (int m = 0; m < 5; m++) x[m] = s[m][k];
+                matvec5(L, x, y);
+                for (int m = 0; m < 5; m++) s[m][k] = y[m];
+            }
+        }
+        int c = j + 1, r = j - n;
+        for (int m = 0; m < 5; m++) {
+            double hv = 0.0;
+            int flag = side == 0 ? closure_P(bl, br, c, r, s[m], ka1, kb1, &hv)
+                                 : closure_M(bl, br, c, r, s[m], ka1, kb1, &hv);
+            if (flag != 0) hv = scheme_flux(scheme, side, hy, s[m], tvlim, tvmax);
+            x[m] = hv;
+        }
+        if (proj) matvec5(R, x, y);
+        else for (int m = 0; m < 5; m++) y[m] = x[m];
+        for (int m = 0; m < 5; m++) hf[m * (n + 1) + j + 1] = y[m];
+    }
+    const int fy = bl && br ? 7 : bl ? 1 : br ? 4 : 0;
+#define FQ(m, q) F[m][(long)((q) + LAP) * fs]
+    for (int i = 0; i < n; i++) {
+        double aj = ajac[(long)(i + LAP) * fs];
+        for (int m = 0; m < 5; m++) {
+            double t;
+            int zero = 0;
+            if (side == 0) {
+                if (fy == 1 && i == 0) {
+                    zero = nonref;
+                    t = FQ(m, 1) - FQ(m, 0);
+                } else if (fy == 4 && i == n - 1) {
+                    double a = FQ(m, n - 1), b = FQ(m, n - 2), cc = FQ(m, n - 3);
+                    t = (a + 0.5 * minmod2(a - b, a - b)) - (b + 0.5 * minmod2(a - b, b - cc));
+                } else {
+                    t = hf[m * (n + 1) + i + 1] - hf[m * (n + 1) + i];
+                }
+            } else {
+                if (fy == 4 && i == n - 1) {
+                    zero = nonref;
+                    t = FQ(m, n - 1) - FQ(m, n - 2);
+                } else if (fy == 1 && i == 0 && dir != 1) {
+                    double a = FQ(m, 0), b = FQ(m, 1), cc = FQ(m, 2);
+                    t = (b - 0.5 * minmod2(cc - b, b - a)) - (a - 0.5 * minmod2(b - a, b - a));
+                } else {
+                    t = hf[m * (n + 1) + i + 1] - hf[m * (n + 1) + i];
+                }
+            }
+            if (zero) du[m][(long)i * ds] += 0;
+            else du[m][(long)i * ds] += -aj * t / h;
+        }
+    }
+#undef FQ
+    free(hf);
+}
+
 /* Generic central derivative with D0 closures, OCFD_CD6_kernel/OCFD_CD8_kernel
  * (src/OCFD_Schemes.cu:96-156) + OCFD_D0bound_scheme_kernel
  * (src/OCFD_bound_Scheme.cu:79-186) + put_d0_kernel (src/OCFD_Schemes.cu:
@@ -1244,6 +1392,31 @@ static void du_invis(orc *o) {
         int hyb = o->c.scheme_invis == OCFD_HYBRIDAUTO;
         int ex = nx + (dir == 0), ey = ny + (dir == 1);
         long gs = dir == 0 ? 1 : dir == 1 ? ex : (long)ex * ey;
+        if (o->c.charteric) {
+            for (int side = 0; side < 2; side++) {
+                const double *F = side == 0 ? o->fp[dir] : o->fm[dir];
+                int nonref = o->c.non_ref[2 * dir + side];
+#pragma omp parallel for collapse(2) schedule(static)
+                for (int b = 0; b < nb; b++)
+                    for (int a = 0; a < na; a++) {
+                        int X = dir == 0 ? 0 : a, Y = dir == 0 ? a : dir == 1 ? 0 : b, Z = dir == 2 ? 0 : b;
+                        long base = IL(o, dir == 0 ? 0 : X + LAP, dir == 1 ? 0 : Y + LAP, dir == 2 ? 0 : Z + LAP);
+                        const double *fl[5], *pr[7];
+                        double *dul[5];
+                        for (int m = 0; m < 5; m++) {
+                            fl[m] = F + m * o->nlap + base;
+                            dul[m] = o->du + m * N + I0(o, X, Y, Z);
+                        }
+                        pr[0] = o->u + base; pr[1] = o->v + base; pr[2] = o->w + base; pr[3] = o->cc + base;
+                        for (int t = 0; t < 3; t++) pr[4 + t] = o->A[ai[dir][t]] + base;
+                        const int *fl_h = NULL;
+                        if (hyb) fl_h = o->flag[dir] + (long)X + (long)ex * ((long)Y + (long)ey * Z);
+                        line_flux_char(o->c.scheme_invis, side, dir, n, fl, fs, pr, o->A[AJAC] + base, h, bl, br,
+                                       nonref, fl_h, gs, o->c.gamma, dul, ds, 5.0, 0.2);
+                    }
+            }
+            continue;
+        }
         for (int side = 0; side < 2; side++) {
             const double *F = side == 0 ? o->fp[dir] : o->fm[dir];
             int nonref = o->c.non_ref[2 * dir + side];
@@ -1592,4 +1765,27 @@ int orc_get_q(orc *o, double *q) {
                 q[p] = (ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy) * aj * aj;
             }
     return 0;
+}
+
+/* Characteristic matrices of a face (pins of the projection). */
+void orc_char_matrices(const double *s0, const double *s1, double gamma, double *L, double *R) {
+    double P[14];
+    char_para(s0, s1, gamma, P);
+    char_matrices(P, L, R);
+}
+
+/* One line of the characteristic flux derivative with contiguous arrays:
+ * F (5 x (n+8)), prim (7 x (n+8)) = u v w c a1 a2 a3, ajac (n+8), flags
+ * (n+1, face j+1/2 at j+1) or NULL; du (5 x n) accumulated. */
+void orc_line_flux_char(int scheme, int side, int dir, int n, const double *F, const double *prim,
+                        const double *ajac, double h, int bl, int br, int nonref, const int *flags, double gamma,
+                        double *du) {
+    const double *f5[5], *p7[7];
+    double *d5[5];
+    for (int m = 0; m < 5; m++) {
+        f5[m] = F + (long)m * (n + 8);
+        d5[m] = du + (long)m * n;
+    }
+    for (int t = 0; t < 7; t++) p7[t] = prim + (long)t * (n + 8);
+    line_flux_char(scheme, side, dir, n, f5, 1, p7, ajac, h, bl, br, nonref, flags, 1, gamma, d5, 1, 5.0, 0.2);
 }

@@ -46,6 +46,13 @@ int orc_get_pressure_lap(orc *o, double *p);
 int orc_get_amu(orc *o, double *amu);
 int orc_get_metrics(orc *o, double *a13);
 int orc_get_flags(orc *o, signed char *fx, signed char *fy, signed char *fz);
+/* Characteristic projection of a face from the u v w c a1 a2 a3 of its two
+ * nodes: left and right eigenvector matrices (5x5, row-major). */
+void orc_char_matrices(const double *s0, const double *s1, double gamma, double *L, double *R);
+/* One line of the CHARTERIC = 1 flux derivative (contiguous arrays). */
+void orc_line_flux_char(int scheme, int side, int dir, int n, const double *F, const double *prim,
+                        const double *ajac, double h, int bl, int br, int nonref, const int *flags, double gamma,
+                        double *du);
 /* Q criterion at the interior points (nz*ny*nx). */
 int orc_get_q(orc *o, double *q);
 

@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
             "orc_get_metrics": (C.c_int, [vp, dp]),
             "orc_get_flags": (C.c_int, [vp, i8, i8, i8]),
             "orc_get_q": (C.c_int, [vp, dp]),
+            "orc_char_matrices": (None, [dp, dp, C.c_double, dp, dp]),
+            "orc_line_flux_char": (None, [C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int,
+                                          C.c_int, C.c_int, C.c_void_p, C.c_double, dp]),
             "orc_face_flux": (C.c_double, [C.c_int, C.c_int, C.c_int, dp, C.c_double, C.c_double]),
             "orc_split_point": (None, [C.c_double] * 10 + [dp]),
             "orc_line_flux_derivative": (None, [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.c_int,
@@ -206,6 +209,29 @@ def line_flux_derivative(scheme, side, f, ajac, h, bound_l=0, bound_r=0, nonref=
         fl = fl_arr.ctypes.data_as(C.c_void_p)
     lib().orc_line_flux_derivative(scheme, side, n, f, aj, h, bound_l, bound_r, nonref, fl, du)
     return du
+
+
+def char_matrices(s0, s1, gamma=1.4):
+    """(L, R) of a face from the u v w c a1 a2 a3 of its two nodes."""
+    L, R = np.empty(25), np.empty(25)
+    lib().orc_char_matrices(np.ascontiguousarray(s0, dtype=np.float64), np.ascontiguousarray(s1, dtype=np.float64),
+                            gamma, L, R)
+    return L.reshape(5, 5), R.reshape(5, 5)
+
+
+def line_flux_char(scheme, side, dir, F, prim, ajac, h, bound_l=0, bound_r=0, nonref=0, flags=None, gamma=1.4):
+    """F (5, n+8) split fluxes at nodes -4..n+3, prim (7, n+8) = u v w c a1 a2
+    a3; returns du (5, n) of the CHARTERIC = 1 flux derivative."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    n = F.shape[1] - 8
+    du = np.zeros(5 * n)
+    fl = None
+    if flags is not None:
+        fl_arr = np.ascontiguousarray(flags, dtype=np.int32)
+        fl = fl_arr.ctypes.data_as(C.c_void_p)
+    lib().orc_line_flux_char(scheme, side, dir, n, F, np.ascontiguousarray(prim, dtype=np.float64),
+                             np.ascontiguousarray(ajac, dtype=np.float64), h, bound_l, bound_r, nonref, fl, gamma, du)
+    return du.reshape(5, n)
 
 
 def line_central(scheme_vis, f, h, bound_l=0, bound_r=0):

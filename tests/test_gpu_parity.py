@@ -46,6 +46,24 @@ def test_rhs_wavy_box_all_schemes(scheme):
     assert (e <= TOL).all(), e
 
 
+@pytest.mark.parametrize("scheme", ["WENO7", "UP7", "WENO5", "WENO7_SYMBO", "NND2", "OMP6_HR", "SCHEME_HYBRIDAUTO"])
+@pytest.mark.parametrize("bounds,nonref", [((1, 0, 0, 1, 1, 1), (1, 0, 0, 1, 0, 0)),
+                                           ((0, 1, 1, 0, 0, 0), (0, 1, 1, 1, 0, 0))])
+def test_rhs_closures_low_high_both(scheme, bounds, nonref):
+    """Boundary closures on the low face, the high face and both, with
+    non-reflecting faces, on a non-periodic curvilinear box."""
+    case = cases.wavy_box(20, 16, 12, scheme=scheme, amp=0.08, periodic=(0, 0, 0))
+    for q in range(6):
+        case.cfg.dif_bound[q] = bounds[q]
+        case.cfg.non_ref[q] = nonref[q]
+    g, o = pair(case)
+    g.rhs(1)
+    o.rhs(1)
+    e = relerr(g.get_du(), o.get_du())
+    report(f"rhs closures {scheme} {bounds}", e)
+    assert (e <= TOL).all(), e
+
+
 @pytest.mark.parametrize("scheme", ["WENO7", "SCHEME_HYBRIDAUTO", "OMP6_HR", "WENO7_SYMBO"])
 def test_steps_wavy_box(scheme):
     case = cases.wavy_box(20, 16, 12, scheme=scheme)

@@ -296,6 +296,88 @@ def test_q_criterion_of_linear_velocity(svis):
     assert np.allclose(o.get_q(), -1.0, rtol=0, atol=1e-10)
 
 
+def _euler_flux(U, nvec, gamma):
+    rho, mu, mv, mw, E = U
+    u, v, w = mu / rho, mv / rho, mw / rho
+    p = (gamma - 1) * (E - 0.5 * rho * (u * u + v * v + w * w))
+    un = u * nvec[0] + v * nvec[1] + w * nvec[2]
+    return np.array([rho * un, mu * un + p * nvec[0], mv * un + p * nvec[1], mw * un + p * nvec[2], (E + p) * un])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_char_matrices_are_inverse_eigenvectors(seed):
+    """The characteristic projection of a face: L R = I, and L J R =
+    diag(un, un, un, un - c, un + c) for J the Jacobian of the Euler flux
+    along n = A/|A| at the face state (u, v, w, c) - every row and column of
+    both matrices is pinned by these."""
+    rng = np.random.default_rng(seed)
+    gamma = 1.4
+    s0 = np.concatenate([rng.normal(0, 0.6, 3), [rng.uniform(0.5, 2.0)], rng.normal(0, 1, 3)])
+    s1 = np.concatenate([rng.normal(0, 0.6, 3), [rng.uniform(0.5, 2.0)], rng.normal(0, 1, 3)])
+    if seed % 2:  # exercise both choices of l
+        s0[6], s1[6] = 3.0, 2.5
+    L, R = O.char_matrices(s0, s1, gamma)
+    assert np.allclose(L @ R, np.eye(5), rtol=0, atol=1e-13)
+    assert np.allclose(R @ L, np.eye(5), rtol=0, atol=1e-13)
+    st = 0.5 * (s0 + s1)
+    u, v, w, c = st[:4]
+    nvec = st[4:] / np.linalg.norm(st[4:])
+    rho = 1.3
+    p = rho * c * c / gamma
+    U = np.array([rho, rho * u, rho * v, rho * w, p / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+    J = np.empty((5, 5))
+    for k in range(5):
+        e = np.zeros(5)
+        e[k] = 1e-6 * max(1.0, abs(U[k]))
+        J[:, k] = (_euler_flux(U + e, nvec, gamma) - _euler_flux(U - e, nvec, gamma)) / (2 * e[k])
+    un = u * nvec[0] + v * nvec[1] + w * nvec[2]
+    D = L @ J @ R
+    assert np.allclose(D, np.diag([un, un, un, un - c, un + c]), rtol=0, atol=1e-6 * (abs(un) + c))
+
+
+@pytest.mark.parametrize("scheme", [capi.WENO7_SYMBO, capi.WENO7_SYMBO_LIM, capi.HYBRIDAUTO])
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("dirn", [0, 1, 2])
+@pytest.mark.parametrize("bounds", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("nonref", [0, 1])
+def test_char_line_is_physical_line_on_linear_fluxes(scheme, side, dirn, bounds, nonref):
+    """On split fluxes linear along the line every reconstruction and closure
+    is a fixed linear combination of the stencil, so projecting with L and
+    mapping back with R changes nothing: the CHARTERIC = 1 derivative equals
+    the physical-space one at every node except the boundary nodes where the
+    reference takes a physical-space difference instead, which on linear data
+    is the exact slope (or 0 under the non-reflecting rule of that path)."""
+    n = 16
+    bl, br = bounds
+    rng = np.random.default_rng(7 + 3 * side + dirn)
+    q = np.arange(-4, n + 4, dtype=float)
+    a = rng.normal(0, 1, 5)
+    b = rng.normal(0, 0.3, 5)
+    F = a[:, None] + b[:, None] * q[None, :]
+    prim = np.empty((7, n + 8))
+    prim[0:3] = rng.normal(0, 0.4, (3, 1)) + 0.05 * np.sin(q)[None, :]
+    prim[3] = 1.0 + 0.1 * np.cos(q)
+    prim[4:7] = rng.normal(0, 1, (3, 1)) + 0.05 * np.cos(0.7 * q)[None, :]
+    ajac = 1.0 + 0.2 * np.sin(0.3 * q)
+    h = 0.1
+    flags = rng.integers(1, 4, n + 1) if scheme == capi.HYBRIDAUTO else None
+    got = O.line_flux_char(scheme, side, dirn, F, prim, ajac, h, bl, br, nonref, flags)
+    want = np.stack([O.line_flux_derivative(scheme, side, F[m], ajac, h, bl, br, 0, flags) for m in range(5)])
+    fy = 7 if bl and br else 1 if bl else 4 if br else 0
+    exact = -ajac[4:n + 4][None, :] * b[:, None] / h
+    if side == 0:
+        if fy == 1:
+            want[:, 0] = 0.0 if nonref else exact[:, 0]
+        if fy == 4:
+            want[:, n - 1] = exact[:, n - 1]
+    else:
+        if fy == 4:
+            want[:, n - 1] = 0.0 if nonref else exact[:, n - 1]
+        if fy == 1 and dirn != 1:
+            want[:, 0] = exact[:, 0]
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
 def test_sutherland_at_reference_temperature():
     c = cases.tgv(8, nsteps=1)
     n = 8
