@@ -274,6 +274,21 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
     // consecutive x
     auto at = [](int v, int n, int a) { return (v * XA + a) * XN + n; };
 
+    // the hybrid flag of this thread's face and the Ajac of its output node
+    // are fetched first, so that their latency hides behind phase 1
+    const bool fthread = tid < XF * XA;
+    const int f_loc = fthread ? tid % XF : 0, a_loc = fthread ? tid / XF : 0;
+    const int jf = s0 - 1 + f_loc, acf = a0 + a_loc;
+    const bool fvalid = fthread && (jf <= n_dir - 1) && (acf < g.ny);
+    int hy = 0;
+    if (SCH == OCFD_HYBRIDAUTO && fvalid) hy = A.flag[gidx(g, jf, acf, k2)];
+    const bool nthread = tid < XS * XA;
+    const int n3 = nthread ? tid % XS : 0, a3 = nthread ? tid / XS : 0;
+    const int i3 = s0 + n3, ac3 = a0 + a3;
+    const bool nvalid = nthread && i3 < n_dir && ac3 < g.ny;
+    const long long q3 = nvalid ? gidx(g, i3, ac3, k2) : 0;
+    const double ajac3 = nvalid ? A.jac[q3] : 0.0;
+
     // ---- phase 1: split of XN x XA nodes into LDS
     {
         const int n = tid % XN, a = tid / XN;
@@ -296,25 +311,15 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
     }
     __syncthreads();
 
-    // ---- phase 2: one half node per thread
+    // ---- phase 2: one half node per thread (face jf+1/2)
     S hp[5], hm[5];
-    int f_loc = 0, a_loc = 0;
-    bool fvalid = false;
-    if (tid < XF * XA) {
-        f_loc = tid % XF; a_loc = tid / XF;
-        const int j = s0 - 1 + f_loc;  // face j+1/2
-        const int ac = a0 + a_loc;
-        fvalid = (j <= n_dir - 1) && (ac < g.ny);
-        if (fvalid) {
-            int hy = 0;
-            if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[gidx(g, j, ac, k2)];
-            const int c = j + 1, r = j - n_dir;
-            face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
-                                     S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
-        }
+    if (fvalid) {
+        const int c = jf + 1, r = jf - n_dir;
+        face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
+                                 S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
     }
     __syncthreads();
-    if (tid < XF * XA) {
+    if (fthread) {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
             sh[(m * XF + f_loc) * XA + a_loc] = fvalid ? hp[m] : S(0);
@@ -325,12 +330,11 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
 
     // ---- phase 3: du at nodes, in the reference's order P then M; the
     // Jacobian factor and the accumulation are in double
-    if (tid < XS * XA) {
-        const int n = tid % XS, a = tid / XS;
-        const int i = s0 + n, ac = a0 + a;
-        if (i < n_dir && ac < g.ny) {
-            const long long q = gidx(g, i, ac, k2);
-            const double ajac = A.jac[q];
+    if (nvalid) {
+        const int n = n3, a = a3, i = i3;
+        {
+            const long long q = q3;
+            const double ajac = ajac3;
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
             const double rh = 1.0 / A.h;
@@ -391,6 +395,16 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
         }
     }
 
+    // hybrid flags of this thread's faces, fetched before phase 1 as well
+    int pre_hy[FPT];
+#pragma unroll
+    for (int t = 0; t < FPT; t++) {
+        pre_hy[t] = 0;
+        const int idx = tid + 256 * t;
+        const int j = s0 - 1 + idx / NA, ac = a0 + idx % NA;
+        if (SCH == OCFD_HYBRIDAUTO && idx < NF * NA && j <= n_dir - 1 && ac < g.nx) pre_hy[t] = A.flag[lin(j, ac)];
+    }
+
     // ---- phase 1: split of NN x NA nodes into LDS
     for (int idx = tid; idx < NN * NA; idx += 256) {
         const int a = idx % NA, n = idx / NA;
@@ -423,8 +437,7 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
         const int j = s0 - 1 + f_loc;  // face j+1/2
         const int ac = a0 + a_loc;
         if (j <= n_dir - 1 && ac < g.nx) {
-            int hy = 0;
-            if (SCH == OCFD_HYBRIDAUTO) hy = A.flag[lin(j, ac)];
+            const int hy = pre_hy[t];
             const int c = j + 1, r = j - n_dir;
             face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
                                      S(A.c.tv_lim), S(A.c.tv_max), hp[t], hm[t]);

@@ -38,6 +38,17 @@ __device__ __forceinline__ float gelu(float x) {
     const float t = tanhf(0.7978845608028654f * (x + 0.044715f * x * x * x));
     return 0.5f * x * (1.0f + t);
 }
+// The same GELU for inference: with y as above, 0.5 x (1 + tanh y) =
+// x - x / (exp(2y) + 1), evaluated with the hardware exp2 and reciprocal
+// (about 1 ulp each, far below the BF16 rounding the activations then get);
+// for large |y| the reciprocal is exactly 0 or 1.  exp(2y) = 2^(x (a + b x^2))
+// with a = 2 sqrt(2/pi) log2(e), b = 0.044715 a.
+__device__ __forceinline__ float gelu_fast(float x) {
+    constexpr float a = 2.0f * 0.7978845608028654f * 1.4426950408889634f;
+    constexpr float b = 0.044715f * a;
+    const float e = __builtin_amdgcn_exp2f(x * (a + b * (x * x)));
+    return x - x * __builtin_amdgcn_rcpf(e + 1.0f);
+}
 __device__ __forceinline__ float gelu_grad(float x) {
     const float t = tanhf(0.7978845608028654f * (x + 0.044715f * x * x * x));
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
@@ -56,9 +67,12 @@ struct FeatIn {
     double p00, rama;
 };
 
-__device__ __forceinline__ double jump(double a, double b) {
-    const double s = a + b;
-    return s != 0.0 ? (b - a) / s : 0.0;
+// FP32 arithmetic with the hardware reciprocal and reciprocal square root
+// (about 1 ulp each): the features are FP32 values that the network reads
+// in BF16, and they stay within a few 1e-7 of an FP64 evaluation.
+__device__ __forceinline__ float jump(float a, float b) {
+    const float s = a + b;
+    return s != 0.f ? (b - a) * __builtin_amdgcn_rcpf(s) : 0.f;
 }
 
 template <typename S>
@@ -66,41 +80,42 @@ __device__ void features(const FeatIn<S>& F, int i, int j, int k, float* out) {
     const Geo& g = F.g;
     const long long q = gidx(g, i, j, k);
     const long long st[3] = {1, (long long)g.ldx, g.sz};
+    const float p00 = float(F.p00);
 #pragma unroll
     for (int dd = 0; dd < 3; dd++) {
-        double p[8], r[8];
+        float p[8], r[8];
 #pragma unroll
         for (int o = 0; o < 8; o++) {  // nodes n-3..n+4
             const long long qo = q + (long long)(o - 3) * st[dd];
-            const double rho = double(F.d[qo]);
+            const float rho = float(F.d[qo]);
             r[o] = rho;
-            p[o] = F.p00 * rho * double(F.T[qo]);
+            p[o] = p00 * rho * float(F.T[qo]);
         }
         // node n is p[3]; intervals m = -1, 0, 1 are (p[2],p[3]), (p[3],p[4]), (p[4],p[5])
-        out[8 * dd + 0] = float(jump(p[2], p[3]));
-        out[8 * dd + 1] = float(jump(p[3], p[4]));
-        out[8 * dd + 2] = float(jump(p[4], p[5]));
-        out[8 * dd + 3] = float(jump(r[2], r[3]));
-        out[8 * dd + 4] = float(jump(r[3], r[4]));
-        out[8 * dd + 5] = float(jump(r[4], r[5]));
-        double mx = fabs(jump(p[0], p[1]));
-        mx = fmax(mx, fabs(jump(p[1], p[2])));
-        mx = fmax(mx, fabs(jump(p[5], p[6])));
-        mx = fmax(mx, fabs(jump(p[6], p[7])));
-        out[8 * dd + 6] = float(mx);
+        out[8 * dd + 0] = jump(p[2], p[3]);
+        out[8 * dd + 1] = jump(p[3], p[4]);
+        out[8 * dd + 2] = jump(p[4], p[5]);
+        out[8 * dd + 3] = jump(r[2], r[3]);
+        out[8 * dd + 4] = jump(r[3], r[4]);
+        out[8 * dd + 5] = jump(r[4], r[5]);
+        float mx = fabsf(jump(p[0], p[1]));
+        mx = fmaxf(mx, fabsf(jump(p[1], p[2])));
+        mx = fmaxf(mx, fabsf(jump(p[5], p[6])));
+        mx = fmaxf(mx, fabsf(jump(p[6], p[7])));
+        out[8 * dd + 6] = mx;
         const double* A1 = F.M[3 * dd];
         const double* A2 = F.M[3 * dd + 1];
         const double* A3 = F.M[3 * dd + 2];
-        double U[2];
+        float U[2];
 #pragma unroll
         for (int m = 0; m < 2; m++) {
             const long long qm = q + m * st[dd];
-            const double a1 = A1[qm], a2 = A2[qm], a3 = A3[qm];
-            U[m] = (a1 * double(F.u[qm]) + a2 * double(F.v[qm]) + a3 * double(F.w[qm])) /
-                   sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+            const float a1 = float(A1[qm]), a2 = float(A2[qm]), a3 = float(A3[qm]);
+            U[m] = (a1 * float(F.u[qm]) + a2 * float(F.v[qm]) + a3 * float(F.w[qm])) *
+                   __builtin_amdgcn_rsqf(a1 * a1 + a2 * a2 + a3 * a3);
         }
-        const double c = sqrt(fmax(double(F.T[q]), 0.0)) * F.rama;
-        out[8 * dd + 7] = float(c != 0.0 ? (U[1] - U[0]) / c : 0.0);
+        const float c = sqrtf(fmaxf(float(F.T[q]), 0.f)) * float(F.rama);
+        out[8 * dd + 7] = c != 0.f ? (U[1] - U[0]) * __builtin_amdgcn_rcpf(c) : 0.f;
     }
 }
 
@@ -129,11 +144,8 @@ __device__ __forceinline__ int jameson_kp(const FeatIn<S>& F, long long q, doubl
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ unsigned short to_bf16(float f) {
-    unsigned int b = __float_as_uint(f);
-    b += 0x7FFFu + ((b >> 16) & 1u);  // round to nearest even
-    return (unsigned short)(b >> 16);
-}
+// round to nearest even (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ unsigned short to_bf16(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
 
 constexpr int LS = 36;  // LDS row stride (bf16 elements) of 32-wide tiles
 
@@ -202,7 +214,7 @@ __global__ void __launch_bounds__(256) k_mlp_sensor(FeatIn<S> F, const float* __
         for (int o = 0; o < 2; o++)
 #pragma unroll
             for (int e = 0; e < 4; e++)
-                X[(16 * t + g4 + e) * LS + 16 * o + r] = to_bf16(gelu(acc[t][o][e] + sb1[16 * o + r]));
+                X[(16 * t + g4 + e) * LS + 16 * o + r] = to_bf16(gelu_fast(acc[t][o][e] + sb1[16 * o + r]));
     __syncthreads();
     wave_layer(X, sW2, lane, acc);
     // output layer: per node sum over the 32 hidden units (16 lanes x 2 tiles)
@@ -213,7 +225,7 @@ __global__ void __launch_bounds__(256) k_mlp_sensor(FeatIn<S> F, const float* __
         for (int e = 0; e < 4; e++) {
             float s = 0.f;
 #pragma unroll
-            for (int o = 0; o < 2; o++) s += gelu(acc[t][o][e] + sb2[16 * o + r]) * sW3[16 * o + r];
+            for (int o = 0; o < 2; o++) s += gelu_fast(acc[t][o][e] + sb2[16 * o + r]) * sW3[16 * o + r];
             s += __shfl_xor(s, 8);
             s += __shfl_xor(s, 4);
             s += __shfl_xor(s, 2);

@@ -2,12 +2,17 @@
 (oracle/mlp_sensor.py): features, the FP32 training gradients, BF16-MFMA
 inference with the confidence-band fallback, and what training gains.
 
-Bars: features to FP32 rounding (1e-6 absolute; they are O(1)); gradients
-to 1e-4 of each parameter block's largest entry (FP32 sums of a 256-sample
-batch against FP64); inference probabilities against a NumPy forward
-that rounds inputs, weights and hidden activations to BF16 as the kernel
-does (the rest is FP32 accumulation order: 1e-4); flags exactly by the
-flag rule.
+Bars: features to FP32 rounding (1e-6 absolute; they are O(1) and the
+kernel evaluates them in FP32 with the hardware reciprocal); gradients to
+1e-4 of each parameter block's largest entry (FP32 sums of a 256-sample
+batch against FP64); inference probabilities against a NumPy forward that
+rounds inputs, weights and hidden activations to BF16 as the kernel does:
+mean difference 2e-5 (FP32 accumulation order), largest 5e-4 (a value that
+sits on a BF16 rounding boundary may round the other way in the kernel,
+whose features and GELU differ from FP64 in the last FP32 bits: one BF16 ulp
+of one hidden unit moves the probability by about 1e-4 with these
+weights); flags exactly by the flag rule on the kernel's own
+probabilities.
 """
 import numpy as np
 import pytest
@@ -102,9 +107,9 @@ def test_inference_matches_bf16_oracle_and_flag_rule():
     a2 = M.gelu(a1 @ W2.T + p["b2"])
     ref = 1.0 / (1.0 + np.exp(-(a2 @ p["W3"] + p["b3"][0])))
     got = prob.reshape(-1)
-    err = np.abs(got - ref).max()
-    print(f"[mlp] inference max |prob - ref| = {err:.2e}", flush=True)
-    assert err <= 1e-4
+    err = np.abs(got - ref)
+    print(f"[mlp] inference |prob - ref| max {err.max():.2e} mean {err.mean():.2e}", flush=True)
+    assert err.max() <= 5e-4 and err.mean() <= 2e-5
     kp = M.jameson_kp(p00 * d[3:-3, 3:-3, 3:-3] * T[3:-3, 3:-3, 3:-3], *case.cfg.hy_dp_intv).reshape(-1)
     want = M.flag_rule(got, kp, lo, hi).reshape(nz, ny, nx)
     assert np.array_equal(fx[:, :, 1:], want)
