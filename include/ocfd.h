@@ -152,6 +152,13 @@ int ocfd_load_flow(ocfd_ctx *ctx, const double *d, const double *u,
  * without it the uniform (1,1,0,1) profile is used when BC_NPARA[2] != 1. */
 int ocfd_set_inlet_profile(ocfd_ctx *ctx, const double *prof4xny);
 
+/* IBC 124 boundary tables (bc_user_Liftbody3d_init,
+ * src/OCFD_boundary_init.c:118-201): inlet5 = d,u,v,w,T at x = 0 as five
+ * arrays of nz*ny (y fastest; the reference's flow-inlet-section.dat, used
+ * when IF_WITHLEADING = 1), upper5 = five arrays of ny*nx (x fastest; its
+ * flow-outboundary.dat, used when IFLAG_UPPERBOUNDARY = 1).  Either may be
+ * NULL.  Call before ocfd_init. */
+int ocfd_set_bc_tables(ocfd_ctx *ctx, const double *inlet5, const double *upper5);
 /* Finish initialisation: prim->cons, boundary-condition set-up, the initial
  * ghost fill, BC and viscosity of NS_solver_real (src/OCFD_init.cu:99-131,
  * src/OCFD_NS_Solver.cu:46-54). */

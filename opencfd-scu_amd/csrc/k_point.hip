@@ -138,16 +138,17 @@ __global__ void k_faces(BCArgsT<S> A) {
         const int side = (int)(t / nyz);
         const long long r = t % nyz;
         i = side ? g.nx - 1 : 0; j = (int)(r % g.ny); k = (int)(r / g.ny);
-        mu = side == 0;
+        mu = (A.mu_faces >> side) & 1;
     } else if ((t -= 2 * nyz) < 2 * nxz) {
         const int side = (int)(t / nxz);
         const long long r = t % nxz;
         i = (int)(r % g.nx); j = side ? g.ny - 1 : 0; k = (int)(r / g.nx);
-        mu = true;
+        mu = (A.mu_faces >> (2 + side)) & 1;
     } else if ((t -= 2 * nxz) < 2 * nxy) {
         const int side = (int)(t / nxy);
         const long long r = t % nxy;
         i = (int)(r % g.nx); j = (int)(r / g.nx); k = side ? g.nz - 1 : 0;
+        mu = (A.mu_faces >> (4 + side)) & 1;
     } else {
         return;
     }
@@ -157,6 +158,87 @@ __global__ void k_faces(BCArgsT<S> A) {
         A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
     }
     p2c(A, q);
+}
+
+// ---------------------------------------------------------------------------
+// IBC 124, bc_user_Liftbody3d (src/OCFD_boundary_Liftbody3D.cu:32-294):
+// inlet (x ghosts and i = 0 from the inlet table), upper face (k = nz-1: free
+// stream at the angle of attack, or the upper table, plus the upper
+// disturbance on u), wall (k = 0: blowing/suction times ht when a
+// disturbance table exists, else no slip; T = Tw with the density from
+// 2nd-order extrapolated pressure, or T and rho extrapolated when Tw <= 0),
+// then the symmetry planes (y ghosts mirrored about j = 0 and j = ny-1, v
+// odd).  One launch per step of the reference, in its order.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void k_bc124_inlet(BCArgsT<S> A) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    const Geo& g = A.g;
+    if (j >= g.ny) return;
+    const long long t = (long long)g.nz * g.ny, o = (long long)k * g.ny + j;
+    for (int i = -4; i <= 0; i++) {
+        const long long q = gidx(g, i, j, k);
+        A.d[q] = S(A.in124[o]);
+        A.u[q] = S(A.in124[t + o]);
+        A.v[q] = S(A.in124[2 * t + o]);
+        A.w[q] = S(A.in124[3 * t + o]);
+        A.T[q] = S(A.in124[4 * t + o]);
+    }
+}
+template <typename S>
+__global__ void k_bc124_upper(BCArgsT<S> A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Geo& g = A.g;
+    if (i >= g.nx) return;
+    const long long q = gidx(g, i, j, g.nz - 1), t = (long long)g.ny * g.nx, o = (long long)j * g.nx + i;
+    const double dist = A.udist124[o];
+    if (A.up124) {  // do_u2d_upper_kernel
+        A.d[q] = S(A.up124[o]);
+        A.u[q] = S(A.up124[t + o] + dist);
+        A.v[q] = S(A.up124[2 * t + o]);
+        A.w[q] = S(A.up124[3 * t + o]);
+        A.T[q] = S(A.up124[4 * t + o]);
+    } else {  // do_u_dist_upper_kernel
+        A.d[q] = S(1.0);
+        A.u[q] = S(A.cos_aoa + dist);
+        A.v[q] = S(0.0);
+        A.w[q] = S(A.sin_aoa);
+        A.T[q] = S(1.0);
+    }
+}
+template <typename S>
+__global__ void k_bc124_wall(BCArgsT<S> A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    const Geo& g = A.g;
+    if (i >= g.nx) return;
+    const long long q0 = gidx(g, i, j, 0), q1 = gidx(g, i, j, 1), q2 = gidx(g, i, j, 2);
+    const long long t = (long long)g.ny * g.nx, o = (long long)j * g.nx + i;
+    if (A.coeff124) {  // do_wall_kernel_*_V
+        A.u[q0] = S(A.coeff124[o] * A.ht);
+        A.v[q0] = S(A.coeff124[t + o] * A.ht);
+        A.w[q0] = S(A.coeff124[2 * t + o] * A.ht);
+    } else {  // do_wall_kernel_*_NV
+        A.u[q0] = S(0.0);
+        A.v[q0] = S(0.0);
+        A.w[q0] = S(0.0);
+    }
+    if (A.tw > 0.0) {
+        A.T[q0] = S(A.tw);
+        A.d[q0] = S((4.0 * double(A.d[q1]) * double(A.T[q1]) - double(A.d[q2]) * double(A.T[q2])) / (3.0 * A.tw));
+    } else {
+        A.T[q0] = S((4.0 * double(A.T[q1]) - double(A.T[q2])) / 3.0);
+        A.d[q0] = S((4.0 * double(A.d[q1]) - double(A.d[q2])) / 3.0);
+    }
+}
+template <typename T>
+__global__ void k_mirror_y(Geo g, T* f, double sign) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y, k = blockIdx.z;
+    if (i >= g.nx) return;
+    // l = 0..3: ghost j = -1-l from interior 1+l; l = 4..7: ghost j = ny+(l-4)
+    // from interior ny-2-(l-4)
+    const int jg = l < 4 ? -1 - l : g.ny + (l - 4);
+    const int js = l < 4 ? 1 + l : g.ny - 2 - (l - 4);
+    f[gidx(g, i, jg, k)] = T(sign * double(f[gidx(g, i, js, k)]));
 }
 
 // ---------------------------------------------------------------------------
@@ -447,6 +529,23 @@ void launch_bc108(const BCArgsT<S>& a, hipStream_t st) {
     hipLaunchKernelGGL(k_bc_wall<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
 }
 
+template <typename T>
+void launch_mirror_y(const Geo& g, T* f, double sign, hipStream_t st) {
+    hipLaunchKernelGGL(k_mirror_y<T>, dim3((g.nx + 63) / 64, 8, g.nz), dim3(64), 0, st, g, f, sign);
+}
+
+template <typename S>
+void launch_bc124(const BCArgsT<S>& a, int sym, hipStream_t st) {
+    const Geo& g = a.g;
+    hipLaunchKernelGGL(k_bc124_inlet<S>, dim3((g.ny + 63) / 64, g.nz), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc124_upper<S>, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_bc124_wall<S>, dim3((g.nx + 63) / 64, g.ny), dim3(64), 0, st, a);
+    if (sym) {
+        S* f[5] = {a.d, a.u, a.v, a.w, a.T};
+        for (int m = 0; m < 5; m++) launch_mirror_y(g, f[m], m == 2 ? -1.0 : 1.0, st);
+    }
+}
+
 template <typename S>
 void launch_face_refresh(const BCArgsT<S>& a, hipStream_t st) {
     const Geo& g = a.g;
@@ -523,6 +622,8 @@ void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, 
     template void launch_fill<S>(const Geo&, S*, int, long long, int, hipStream_t);                          \
     template void launch_stage<S>(const StageArgsT<S>&, hipStream_t);                                        \
     template void launch_bc108<S>(const BCArgsT<S>&, hipStream_t);                                           \
+    template void launch_bc124<S>(const BCArgsT<S>&, int, hipStream_t);                                      \
+    template void launch_mirror_y<S>(const Geo&, S*, double, hipStream_t);                                   \
     template void launch_face_refresh<S>(const BCArgsT<S>&, hipStream_t);                                    \
     template void launch_sensor<S>(const Geo&, const Consts&, const S*, const S*, int8_t*, int8_t*, int8_t*, \
                                    hipStream_t);                                                              \

@@ -20,8 +20,8 @@
 // into du are double.
 //
 // Both use 2.5-D blocking: a block owns a TX x TY tile of (x,y) columns and
-// marches through a chunk of z planes (32 on the FP64 path, 16 on the FP32
-// path, which then has twice the blocks).  x and y neighbours come from the
+// marches through a chunk of z planes (gradients: 32 on the FP64 path, 8 on
+// the FP32 path; divergence: 16).  x and y neighbours come from the
 // current plane staged in LDS with its 4-point halo; z neighbours come from a
 // 9-plane queue each thread keeps in registers for its own column.
 #include "ocfd_dev.h"
@@ -33,7 +33,7 @@ namespace {
 #define OCFD_VISC_KZ_D 32  // z planes per block, FP64 path
 #endif
 #ifndef OCFD_VISC_KZ_F
-#define OCFD_VISC_KZ_F 16  // z planes per block, FP32 path (more blocks)
+#define OCFD_VISC_KZ_F 8  // z planes per block, FP32 path (measured best of 8, 16, 32)
 #endif
 #ifndef OCFD_VISC_TX
 #define OCFD_VISC_TX 32  // x columns per block
@@ -45,8 +45,17 @@ namespace {
 #define OCFD_VISC_PREFETCH 1  // fetch plane k+1 while plane k is computed
 #endif
 constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
-template <typename S> struct KZ_ { static constexpr int v = OCFD_VISC_KZ_D; };
-template <> struct KZ_<float> { static constexpr int v = OCFD_VISC_KZ_F; };
+#ifndef OCFD_VDIV_KZ_D
+#define OCFD_VDIV_KZ_D 16  // z planes per block of the divergence, FP64 path
+#endif
+#ifndef OCFD_VDIV_KZ_F
+#define OCFD_VDIV_KZ_F 16  // z planes per block of the divergence, FP32 path
+#endif
+// z planes per block: KZ_<S, false> for the gradient/flux kernel, KZ_<S,
+// true> for the divergence (a lighter kernel at higher occupancy, measured
+// faster with more, shorter chunks)
+template <typename S, bool DIV = false> struct KZ_ { static constexpr int v = DIV ? OCFD_VDIV_KZ_D : OCFD_VISC_KZ_D; };
+template <bool DIV> struct KZ_<float, DIV> { static constexpr int v = DIV ? OCFD_VDIV_KZ_F : OCFD_VISC_KZ_F; };
 constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
 
 // Central derivative at interior index i (0..n-1) from s[0..8] = values at
@@ -82,9 +91,9 @@ struct Tile25 {
     int i0, j0, i, j, kbeg, kend;
     bool own;
 };
-template <typename S>
+template <typename S, bool DIV>
 __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
-    constexpr int KZ = KZ_<S>::v;
+    constexpr int KZ = KZ_<S, DIV>::v;
     Tile25 t;
     const Blk B = xcd_block();
     t.i0 = B.x * TX;
@@ -107,7 +116,7 @@ template <int SV, typename S>
 __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
     constexpr int PW = TX + 8, PN = PW * (TY + 8), NSL = (PN + NT - 1) / NT;
     const Geo& g = A.g;
-    const Tile25 t = tile_of<S>(g);
+    const Tile25 t = tile_of<S, false>(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
@@ -246,7 +255,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     constexpr int XW = TX + 8, XN = TY * XW, YN = (TY + 8) * TX;
     constexpr int NSX = (XN + NT - 1) / NT, NSY = (YN + NT - 1) / NT;
     const Geo& g = A.g;
-    const Tile25 t = tile_of<S>(g);
+    const Tile25 t = tile_of<S, true>(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
@@ -392,9 +401,9 @@ __global__ void __launch_bounds__(256) k_qcrit(ViscArgsT<S> A, double* __restric
     qout[qi] = (ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy) * aj * aj;
 }
 
-template <typename S>
+template <typename S, bool DIV>
 dim3 grid25(const Geo& g) {
-    constexpr int KZ = KZ_<S>::v;
+    constexpr int KZ = KZ_<S, DIV>::v;
     return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + KZ - 1) / KZ);
 }
 
@@ -402,8 +411,8 @@ dim3 grid25(const Geo& g) {
 
 template <typename S>
 void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), grid25<S>(a.g), dim3(TX, TY), 0, st, a);
-    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), grid25<S>(a.g), dim3(TX, TY), 0, st, a);
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
+    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
 }
 // Flux components per divergence launch: one on the FP64 path (a quarter of
 // the registers: 4 waves per SIMD instead of 2, measured faster), all four on
@@ -415,8 +424,8 @@ int visc_div_components() { return NC_<S>::v; }
 template <typename S>
 void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st) {
     constexpr int NC = NC_<S>::v;
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
-    else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), grid25<S>(a.g), dim3(TX, TY), 0, st, a, c0);
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), (grid25<S, true>(a.g)), dim3(TX, TY), 0, st, a, c0);
+    else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), (grid25<S, true>(a.g)), dim3(TX, TY), 0, st, a, c0);
 }
 template <typename S>
 void launch_qcrit(const ViscArgsT<S>& a, double* q, hipStream_t st) {

@@ -72,6 +72,9 @@ struct ocfd_ctx {
     float mlp_lo = 0.2f, mlp_hi = 0.8f;
     double* diag_part = nullptr;  // 2 * diag_sum_blocks partial sums
     int* diag_scan = nullptr;     // 2 * DIAG_SCAN_BLOCKS counts
+    double* b124 = nullptr;       // IBC 124 tables: inlet [5][nz][ny], upper [5][ny][nx], coeff [3][ny][nx], udist [ny][nx]
+    std::vector<double> in124_host, up124_host;  // caller tables (flow-inlet-section.dat, flow-outboundary.dat)
+    bool v_dist_need = false;     // IBC 124 wall disturbance present
     double* avg = nullptr;        // time averages of d,u,v,w,T (analysis style 101)
     int avg_n = 0;                // samples in them (Istep_average)
     double avg_tt = 0.0;          // their accumulated time (tt_average)
@@ -297,6 +300,23 @@ BCArgsT<S> bc_args(ocfd_ctx* x) {
     a.epsl = x->cfg.bc_rpara[1];
     a.tw = x->cfg.bc_rpara[0];
     a.wall_not_normal = x->cfg.bc_npara[3];
+    // the viscosity follows the primitives the boundary condition changes:
+    // x = 0, y = 0 and y = ny-1 for IBC 108; x = 0, z = 0 and z = nz-1 for 124
+    a.mu_faces = x->cfg.ibc == 108 ? 0x0d : x->cfg.ibc == 124 ? 0x31 : 0;
+    a.in124 = a.up124 = a.coeff124 = a.udist124 = nullptr;
+    a.sin_aoa = a.cos_aoa = 0.0;
+    if (x->cfg.ibc == 124 && x->b124) {
+        const Geo& g = x->g;
+        const long long nyz = (long long)g.ny * g.nz, nxy = (long long)g.nx * g.ny;
+        a.in124 = x->b124;
+        a.up124 = x->cfg.bc_npara[2] == 1 ? x->b124 + 5 * nyz : nullptr;
+        a.coeff124 = x->v_dist_need ? x->b124 + 5 * nyz + 5 * nxy : nullptr;
+        a.udist124 = x->b124 + 5 * nyz + 8 * nxy;
+        const double aoa = x->cfg.bc_rpara[0] * 3.141592653589793 / 180;
+        a.sin_aoa = std::sin(aoa);
+        a.cos_aoa = std::cos(aoa);
+        a.tw = x->cfg.bc_rpara[1];
+    }
     return a;
 }
 
@@ -315,6 +335,18 @@ int apply_bc(ocfd_ctx* x) {
         }
         a.ht = ht;
         TRY(timed(x, "bc108", [&] { launch_bc108(a, x->st); }));
+    } else if (x->cfg.ibc == 124) {
+        // bc_user_Liftbody3d (src/OCFD_boundary_Liftbody3D.cu:221-229): the
+        // wall-disturbance time factor has no phase here
+        double ht = 0.0;
+        const double beta = x->cfg.bc_rpara[4];
+        if (beta > 0.) {
+            for (int m = 0; m < x->cfg.bc_npara[4] && m < 64; m++) ht = ht + x->TM[m] * std::sin((m + 1) * beta * x->tt);
+        } else {
+            ht = 1.;
+        }
+        a.ht = ht;
+        TRY(timed(x, "bc124", [&] { launch_bc124(a, x->cfg.bc_npara[0] == 1, x->st); }));
     } else {
         a.amu = nullptr;  // no face changes primitives: mu is already current
     }
@@ -546,6 +578,155 @@ int ibc108_init(ocfd_ctx* x) {
     return 0;
 }
 
+// bc_user_Liftbody3d_init (src/OCFD_boundary_init.c:95-318) with the
+// parameters of bc_parameter (:50-71): the inlet table from the caller (the
+// reference's flow-inlet-section.dat) when IF_WITHLEADING = 1, else from the
+// loaded flow at i = 0; the upper table from the caller when
+// IFLAG_UPPERBOUNDARY = 1; the wall disturbance (a multi-wave blowing/suction
+// pattern along y when MZMAX != 0, else seeded noise on the wall points whose
+// x lies in [WALL_DIS_BEGIN, WALL_DIS_END], along the mesh line normal to the
+// wall) and the upper disturbance of u; the time factors TM.  The reference
+// seeds rand() with the clock, so no two of its runs agree; glibc rand() from
+// its default seed is used here, which makes runs repeatable.
+template <typename S>
+int ibc124_init(ocfd_ctx* x) {
+    const ocfd_config& C = x->cfg;
+    const Geo& g = x->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long nyz = (long long)ny * nz, nxy = (long long)nx * ny;
+    const int withleading = C.bc_npara[1], upper_mode = C.bc_npara[2], MZMAX = C.bc_npara[3], MTMAX = C.bc_npara[4];
+    const double EPSL_WALL = C.bc_rpara[2], EPSL_UPPER = C.bc_rpara[3];
+    const double DB = C.bc_rpara[5], DE = C.bc_rpara[6];
+    const double PI = 3.141592653589793;
+    if (x->mesh.size() != 3 * (size_t)nx * ny * nz) return fail(OCFD_ERR_STATE, "IBC 124 needs the mesh coordinates");
+    std::vector<double> tab((size_t)(5 * nyz + 9 * nxy), 0.0);
+    double* in = tab.data();
+    double* up = in + 5 * nyz;
+    double* coeff = up + 5 * nxy;
+    double* udist = coeff + 3 * nxy;
+    if (withleading == 1) {
+        if (x->in124_host.size() != (size_t)(5 * nyz))
+            return fail(OCFD_ERR_STATE, "IBC 124 with IF_WITHLEADING = 1 needs the inlet table (ocfd_set_bc_tables)");
+        std::copy(x->in124_host.begin(), x->in124_host.end(), in);
+    } else {
+        const View<S> V = view<S>(x);
+        const S* f[5] = {V.d, V.u, V.v, V.w, V.T};
+        for (int m = 0; m < 5; m++) {
+            std::vector<S> h((size_t)x->fs);
+            HIPCK(hipStreamSynchronize(x->st));
+            HIPCK(hipMemcpy(h.data(), f[m], sizeof(S) * (size_t)x->fs, hipMemcpyDeviceToHost));
+            for (int k = 0; k < nz; k++)
+                for (int j = 0; j < ny; j++) in[m * nyz + (long long)k * ny + j] = double(h[(size_t)gidx(g, 0, j, k)]);
+        }
+    }
+    if (upper_mode == 1) {
+        if (x->up124_host.size() != (size_t)(5 * nxy))
+            return fail(OCFD_ERR_STATE, "IBC 124 with IFLAG_UPPERBOUNDARY = 1 needs the upper table (ocfd_set_bc_tables)");
+        std::copy(x->up124_host.begin(), x->up124_host.end(), up);
+    }
+    const double* X = x->mesh.data();
+    const double* Y = X + (size_t)nx * ny * nz;
+    const double* Z = Y + (size_t)nx * ny * nz;
+    auto at = [&](int i, int j, int k) { return ((size_t)k * ny + j) * nx + i; };
+    srand(1);
+    bool need = false;
+    if (MZMAX != 0) {
+        // get_xy_blow_suction_multiwave (:320-372) along x at j = k = 0 and y
+        need = true;
+        const int MZ1 = std::abs(MZMAX);
+        std::vector<double> faiz(MZ1 + 1), zl(MZ1 + 1), fx(nx), gz(ny);
+        double ztmp = 0.;
+        for (int k = 0; k < MZ1; k++) {
+            faiz[k] = rand() / (double)RAND_MAX;
+            zl[k] = (k == 0) ? 1. : zl[k - 1] / 1.25;
+            ztmp = ztmp + zl[k];
+        }
+        for (int k = 0; k < MZ1; k++) zl[k] = zl[k] / ztmp;
+        for (int i = 0; i < nx; i++) {
+            const double xa = X[at(i, 0, 0)];
+            if (xa >= DB && xa <= DE) {
+                const double seta = 10. * PI * (xa - DB) / (DB - DE);
+                fx[i] = 4. / std::sqrt(27.) * std::sin(seta) * (1. - std::cos(seta));
+            } else {
+                fx[i] = 0.;
+            }
+        }
+        for (int j = 0; j < ny; j++) {
+            gz[j] = 0.;
+            const double seta = ((double)j) / ny;
+            if (MZMAX > 0) {
+                for (int m = 0; m < MZMAX; m++) gz[j] = gz[j] + zl[m] * std::sin(2. * PI * (m + 1) * (seta + faiz[m]));
+            } else {
+                gz[j] = std::sin(-2. * PI * MZMAX * seta);
+            }
+        }
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const double vdw = fx[i] * gz[j];
+                const size_t o = (size_t)j * nx + i;
+                coeff[o] = 0;
+                coeff[nxy + o] = EPSL_WALL * vdw * std::sin(2 * PI / ny * j);
+                coeff[2 * nxy + o] = EPSL_WALL * vdw * std::cos(2 * PI / ny * j);
+            }
+    } else {
+        std::vector<double> tv(nxy), tu(nxy);
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double r = (rand() / (double)RAND_MAX - 0.5) * 2.0;
+                tv[(size_t)j * nx + i] = EPSL_WALL * r;
+                r = (rand() / (double)RAND_MAX - 0.5) * 2.0;
+                tu[(size_t)j * nx + i] = EPSL_UPPER * r;
+            }
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const size_t o = (size_t)j * nx + i;
+                const double xw = X[at(i, j, 0)];
+                double vdw = 0.0;
+                if (xw >= DB && xw <= DE) {
+                    need = true;
+                    vdw = tv[o];
+                }
+                udist[o] = tu[o];
+                const double xn = X[at(i, j, 1)] - X[at(i, j, 0)], yn = Y[at(i, j, 1)] - Y[at(i, j, 0)];
+                const double zn = Z[at(i, j, 1)] - Z[at(i, j, 0)];
+                const double sn = std::sqrt(xn * xn + yn * yn + zn * zn);
+                coeff[o] = vdw * xn / sn;
+                coeff[nxy + o] = vdw * yn / sn;
+                coeff[2 * nxy + o] = vdw * zn / sn;
+            }
+    }
+    x->v_dist_need = need;
+    // get_fait_multifrequancy (:501-517)
+    double Ttmp = 0.;
+    for (int k = 0; k < MTMAX && k < 64; k++) {
+        x->fait[k] = rand() / (double)RAND_MAX;
+        x->TM[0] = 1.;
+        if (k > 0) x->TM[k] = x->TM[k - 1] / 1.25;
+        Ttmp = Ttmp + x->TM[k];
+    }
+    for (int k = 0; k < MTMAX && k < 64; k++) x->TM[k] = x->TM[k] / Ttmp;
+    int rc = 0;
+    if (!x->b124) {
+        x->b124 = (double*)alloc_bytes(x, sizeof(double) * tab.size(), rc);
+        if (rc) return rc;
+    }
+    HIPCK(hipMemcpyAsync(x->b124, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, x->st));
+    HIPCK(hipStreamSynchronize(x->st));
+    return 0;
+}
+
+// Metrics of the symmetry planes of IBC 124 (boundary_Jac3d_Liftbody_Ajac,
+// src/OCFD_Comput_Jacobian3d.cu:619-662): y ghosts mirrored, with the sign of
+// a vector's y component (and of the y row of the cofactors) flipped.
+int ibc124_mirror_metrics(ocfd_ctx* x) {
+    if (x->cfg.ibc != 124 || x->cfg.bc_npara[0] != 1) return 0;
+    // M order: Akx Aky Akz Aix Aiy Aiz Asx Asy Asz Ajac
+    const double sign[M_COUNT] = {1, -1, 1, -1, 1, -1, 1, -1, 1, 1};
+    for (int m = 0; m < M_COUNT; m++)
+        TRY(timed(x, "mirror", [&] { launch_mirror_y(x->g, x->M[m], sign[m], x->st); }));
+    return 0;
+}
+
 // ocfd_init body: pri_to_cons over the interior (src/OCFD_init.cu:99-110),
 // the IBC 108 set-up, then the NS_solver_real prologue
 // (src/OCFD_NS_Solver.cu:46-54): ghost fill, BC, viscosity everywhere.
@@ -554,6 +735,7 @@ int init_fields(ocfd_ctx* x) {
     BCArgsT<S> a = bc_args<S>(x);
     TRY(timed(x, "prim2cons", [&] { launch_p2c_all(a, x->st); }));
     if (x->cfg.ibc == 108) TRY(ibc108_init(x));
+    if (x->cfg.ibc == 124) TRY(ibc124_init<S>(x));
     TRY(fill_prims<S>(x));
     TRY(apply_bc<S>(x));
     a = bc_args<S>(x);
@@ -589,7 +771,11 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     if (C.scheme_vis != OCFD_CD6 && C.scheme_vis != OCFD_CD8) return fail(OCFD_ERR_CONFIG, "SCHEME_VIS not recognised");
     if (C.scheme_invis == OCFD_HYBRIDAUTO && C.hy_style != 2)
         return fail(OCFD_ERR_UNSUPPORTED, "hybrid_auto: only HY_STYLE = 2 (Jameson sensor) is supported");
-    if (C.ibc != 0 && C.ibc != 108) return fail(OCFD_ERR_UNSUPPORTED, "IBC: only 0 and 108 are supported");
+    if (C.ibc != 0 && C.ibc != 108 && C.ibc != 124)
+        return fail(OCFD_ERR_UNSUPPORTED, "IBC: only 0, 108 and 124 are supported");
+    // INIT_STAT = 0 runs the reference's built-in test geometry and a
+    // different boundary routine (bc_user_Liftbody3d_simple), not provided
+    if (C.ibc == 124 && C.init_stat != 1) return fail(OCFD_ERR_UNSUPPORTED, "IBC 124 needs INIT_STAT = 1");
     if (C.precision != OCFD_PREC_FP64 && C.precision != OCFD_PREC_MIXED)
         return fail(OCFD_ERR_CONFIG, "PRECISION: FP64 or MIXED");
     const long long npts = (long long)(C.nx + 8) * (C.ny + 8) * (C.nz + 8);
@@ -728,6 +914,12 @@ int ocfd_load_mesh(ocfd_ctx* x, const double* X, const double* Y, const double* 
     const double* src[3] = {X, Y, Z};
     for (int q = 0; q < 3; q++) TRY(upload_interior(x, crd + q * x->fs, src[q], 1.0));
     TRY(fill_periodic(x, crd, 3, "metric_fill"));
+    if (C.ibc == 124 && C.bc_npara[0] == 1) {
+        // boundary_Jac3d_Axx (src/OCFD_Comput_Jacobian3d.cu:592-617): the
+        // coordinates of the symmetry planes' ghosts, y odd
+        for (int q = 0; q < 3; q++)
+            TRY(timed(x, "mirror", [&] { launch_mirror_y(g, crd + q * x->fs, q == 1 ? -1.0 : 1.0, x->st); }));
+    }
     // 9 derivatives into gradient fields 0..8: xi xj xk yi yj yk zi zj zk
     for (int q = 0; q < 3; q++)
         for (int dir = 0; dir < 3; dir++)
@@ -736,6 +928,7 @@ int ocfd_load_mesh(ocfd_ctx* x, const double* X, const double* Y, const double* 
             }));
     TRY(timed(x, "cofactors", [&] { launch_cofactors(g, x->grad, x->fs, x->M, x->st); }));
     TRY(fill_periodic(x, x->M[0], M_COUNT, "metric_fill"));
+    TRY(ibc124_mirror_metrics(x));
     HIPCK(hipStreamSynchronize(x->st));
     // host copies for the IBC 108 set-up and the Jacobi writer
     const size_t nxy = (size_t)g.nx * g.ny;
@@ -759,6 +952,7 @@ int ocfd_load_jacobi(ocfd_ctx* x, const double* a13) {
     // file order Axx Ayy Azz Akx Aky Akz Aix Aiy Aiz Asx Asy Asz Ajac
     for (int m = 0; m < M_COUNT; m++) TRY(upload_interior(x, x->M[m], a13 + (3 + m) * n, 1.0));
     TRY(fill_periodic(x, x->M[0], M_COUNT, "metric_fill"));
+    TRY(ibc124_mirror_metrics(x));  // src/OCFD_init.cu:116-122
     HIPCK(hipStreamSynchronize(x->st));
     const double* X = a13;
     const double* Z = a13 + 2 * n;
@@ -793,6 +987,14 @@ int ocfd_set_inlet_profile(ocfd_ctx* x, const double* prof) {
     if (!x || !prof) return fail(OCFD_ERR_ARG, "null argument");
     x->inlet.assign(prof, prof + 4 * x->g.ny);
     x->have_inlet = true;
+    return 0;
+}
+
+int ocfd_set_bc_tables(ocfd_ctx* x, const double* inlet5, const double* upper5) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    const long long nyz = (long long)x->g.ny * x->g.nz, nxy = (long long)x->g.nx * x->g.ny;
+    if (inlet5) x->in124_host.assign(inlet5, inlet5 + 5 * nyz);
+    if (upper5) x->up124_host.assign(upper5, upper5 + 5 * nxy);
     return 0;
 }
 
@@ -1665,6 +1867,22 @@ void init(void) {
         }
         std::fclose(fh);
         SHIM(ocfd_set_inlet_profile(g_ctx, prof.data()));
+    }
+    if (g_cfg.ibc == 124) {
+        // bc_user_Liftbody3d_init (src/OCFD_boundary_init.c:118-201): five
+        // records of NZ*NY (inlet) / NY*NX (upper) values
+        std::vector<double> in, up;
+        if (g_cfg.bc_npara[1] == 1) {
+            std::printf("read inlet boundary data: flow-inlet-section.dat\n");
+            in.resize((size_t)5 * g.ny * g.nz);
+            SHIM(ocfd_read_records("flow-inlet-section.dat", g.ny, g.nz, 1, 5, in.data()));
+        }
+        if (g_cfg.bc_npara[2] == 1) {
+            std::printf("read upper boundary data: flow-outboundary.dat\n");
+            up.resize((size_t)5 * g.nx * g.ny);
+            SHIM(ocfd_read_records("flow-outboundary.dat", g.nx, g.ny, 1, 5, up.data()));
+        }
+        SHIM(ocfd_set_bc_tables(g_ctx, in.empty() ? nullptr : in.data(), up.empty() ? nullptr : up.data()));
     }
     SHIM(ocfd_init(g_ctx));
     std::printf("init ok\n\n");

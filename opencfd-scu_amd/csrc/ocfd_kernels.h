@@ -123,8 +123,21 @@ struct BCArgsT {
     int x_begin;
     double ht, epsl, tw;
     int wall_not_normal;
+    int mu_faces;                          // viscosity refreshed on faces: bit 2*dir (low), 2*dir+1 (high)
+    // IBC 124 (lifting body): inlet [5][nz][ny], upper [5][ny][nx] (or null:
+    // free stream), wall disturbance [3][ny][nx] (or null: none), upper
+    // disturbance [ny][nx]
+    const double *in124, *up124, *coeff124, *udist124;
+    double sin_aoa, cos_aoa;
 };
 template <typename S> void launch_bc108(const BCArgsT<S>& a, hipStream_t st);
+// IBC 124 (bc_user_Liftbody3d, src/OCFD_boundary_Liftbody3D.cu:185-294):
+// inlet, upper face, wall, then the symmetry planes when sym != 0.
+template <typename S> void launch_bc124(const BCArgsT<S>& a, int sym, hipStream_t st);
+// Mirror of one LAP field into the y ghost layers about j = 0 and j = ny-1,
+// times sign, over the interior x and z (boundary_Jac3d_kernal_y_l/r,
+// src/OCFD_Comput_Jacobian3d.cu:429-448; do_symmetry_kernel_m/p).
+template <typename T> void launch_mirror_y(const Geo& g, T* f, double sign, hipStream_t st);
 template <typename S> void launch_face_refresh(const BCArgsT<S>& a, hipStream_t st);
 template <typename S> void launch_p2c_all(const BCArgsT<S>& a, hipStream_t st);
 template <typename S> void launch_amu_all(const BCArgsT<S>& a, hipStream_t st);

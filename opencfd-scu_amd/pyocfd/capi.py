@@ -172,6 +172,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "ocfd_load_jacobi": (C.c_int, [vp, dp]),
         "ocfd_load_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.c_int, C.c_double]),
         "ocfd_set_inlet_profile": (C.c_int, [vp, dp]),
+        "ocfd_set_bc_tables": (C.c_int, [vp, C.c_void_p, C.c_void_p]),
         "ocfd_init": (C.c_int, [vp]),
         "ocfd_rhs": (C.c_int, [vp, C.c_int]),
         "ocfd_step": (C.c_int, [vp, C.c_int]),
@@ -383,6 +384,14 @@ class Solver:
 
     def set_inlet_profile(self, prof):
         _check(self.lib.ocfd_set_inlet_profile(self.h, self._flat(prof)), "set_inlet_profile")
+
+    def set_bc_tables(self, inlet5=None, upper5=None):
+        """IBC 124 tables: inlet (5, nz, ny), upper (5, ny, nx); either None."""
+        a = None if inlet5 is None else self._flat(inlet5)
+        b = None if upper5 is None else self._flat(upper5)
+        _check(self.lib.ocfd_set_bc_tables(self.h, None if a is None else a.ctypes.data_as(C.c_void_p),
+                                           None if b is None else b.ctypes.data_as(C.c_void_p)), "set_bc_tables")
+        self._keep = (a, b)
 
     def init(self):
         _check(self.lib.ocfd_init(self.h), "ocfd_init")

@@ -54,6 +54,8 @@ class Case:
         solver.load_flow(*self.prims, istep=0, tt=0.0)
         if self.inlet is not None:
             solver.set_inlet_profile(self.inlet)
+        if "inlet5" in self.extra or "upper5" in self.extra:
+            solver.set_bc_tables(self.extra.get("inlet5"), self.extra.get("upper5"))
         if init:
             solver.init()
         return solver
@@ -71,6 +73,11 @@ class Case:
         else:
             capi.write_records(d / "OCFD3d-Jacobi.dat", list(self.jacobi))
         capi.write_flow_file(d / "opencfd.dat", self.prims, 0, 0.0)
+        if "inlet5" in self.extra:
+            # one record of nz*ny values per variable (y fastest)
+            capi.write_records(d / "flow-inlet-section.dat", [a_[None] for a_ in self.extra["inlet5"]])
+        if "upper5" in self.extra:
+            capi.write_records(d / "flow-outboundary.dat", [a_[None] for a_ in self.extra["upper5"]])
         if self.inlet is not None:
             ny = self.cfg.ny
             y = self.extra.get("inlet_y", np.arange(ny, dtype=float))
@@ -385,6 +392,65 @@ def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=
         filters=filters, precision=precision)
     return Case("ramp%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z), inlet=inlet,
                 extra={"inlet_y": ycol, "nsteps": nsteps})
+
+
+# --------------------------------------------------------------------------
+# Lifting body / body of revolution (IBC 124)
+# --------------------------------------------------------------------------
+def liftbody(nx=48, ny=24, nz=20, ma=2.0, re=2000.0, aoa=2.0, tw=1.5, delta=0.15, r_out=2.0, cfl=0.4,
+             scheme="WENO7_SYMBO", mzmax=0, mtmax=5, beta=0.3, upper=0, withleading=0, nsteps=10,
+             precision=capi.PREC_FP64, seed=SEED) -> Case:
+    """Half of a body of revolution, the geometry IBC 124 is written for: x
+    along the axis, (y, z) = r (cos t, sin t) with t from pi/2 to -pi/2, so
+    both j = 0 and j = ny-1 lie on the symmetry plane y = 0; the wall is r = 1
+    (k = 0, with a gentle bump along x so that all metrics vary) and the upper
+    face r = r_out (k = nz-1, tanh clustering at the wall).  Free stream at
+    angle of attack aoa (u = cos, w = sin) with a wall layer of thickness
+    delta, seeded noise.  BC_NPARA = IF_SYMMETRY 1, IF_WITHLEADING,
+    IFLAG_UPPERBOUNDARY, MZMAX, MTMAX; BC_RPARA = AOA, TW (<= 0: adiabatic),
+    EPSL_WALL, EPSL_UPPER, BETA, WALL_DIS_BEGIN, WALL_DIS_END.  With
+    withleading / upper set, case.extra holds the inlet / upper tables."""
+    gamma = 1.4
+    rng = np.random.default_rng(seed)
+    xi = np.linspace(0.0, 1.0, nx)
+    th = 0.5 * np.pi - np.pi * np.arange(ny) / (ny - 1)  # decreasing: a right-handed grid
+    s = _tanh_stretch(nz, 0.01)
+    K, J, I = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r0 = 1.0 + 0.05 * np.sin(np.pi * xi[I]) ** 2
+    r = r0 + (r_out - r0) * s[K]
+    x = xi[I] + 0.02 * s[K] * np.sin(2 * np.pi * xi[I])
+    y = r * np.cos(th[J])
+    z = r * np.sin(th[J])
+    y[:, 0, :] = 0.0
+    y[:, -1, :] = 0.0
+    a = math.radians(aoa)
+    eta = np.clip((r - r0) / delta, 0.0, None)
+    f = np.tanh(2.0 * eta)
+    T_wall = tw if tw > 0 else 1.0 + 0.5 * (gamma - 1) * ma * ma * 0.85
+    T = T_wall + (1.0 - T_wall) * f
+    d = 1.0 / T
+    noise = [1e-3 * rng.standard_normal(x.shape) * f * (1 - f + 0.1) for _ in range(3)]
+    u = math.cos(a) * f + noise[0]
+    v = noise[1]
+    w = math.sin(a) * f + noise[2]
+    h = min(1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    dr = np.diff(r, axis=0).min()
+    dt = float(cfl * min(dr, 1.0 / (nx - 1), np.pi / (ny - 1)) / (1.0 + math.sqrt(T.max()) / ma) * 0.5)
+    cfg = capi.make_config(
+        nx=nx, ny=ny, nz=nz, iperiodic=(0, 0, 0), jac_bound=(1, 0, 1),
+        dif_bound=(0, 1, 0, 0, 1, 1), non_ref=(0, 1, 0, 0, 0, 1), scheme_invis=scheme, scheme_vis="CD6",
+        hy_style=2, hy_dp_intv=(0.01, 0.2), re=re, ama=ma, gamma=gamma, pr=0.72, t_ref=288.15, epsl_sw=0.0,
+        dt=dt, end_time=end_time_for(nsteps, dt), ibc=124,
+        bc_npara=(1, withleading, upper, mzmax, mtmax),
+        bc_rpara=(aoa, tw, 1e-3, 1e-4, beta, 0.2, 0.6),
+        precision=precision)
+    extra = {"nsteps": nsteps}
+    if withleading:
+        extra["inlet5"] = np.stack([a_[:, :, 0] for a_ in (d, u, v, w, T)]) * (1.0 + 1e-3)
+    if upper:
+        extra["upper5"] = np.stack([np.full((ny, nx), 1.0), np.full((ny, nx), math.cos(a)), np.zeros((ny, nx)),
+                                    np.full((ny, nx), math.sin(a)), np.full((ny, nx), 1.0)]) * (1.0 + 2e-3)
+    return Case("liftbody%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z), extra=extra)
 
 
 # --------------------------------------------------------------------------

@@ -56,6 +56,11 @@ struct orc {
     double TM[64], fait[64];
     int x_begin;
     int have_profile;
+    /* IBC 124: inlet [5][nz][ny], upper [5][ny][nx], wall coefficients
+     * [3][ny][nx], upper disturbance [ny][nx]; caller tables */
+    double *in124, *up124, *coeff124, *udist124;
+    double *in124_user, *up124_user;
+    int v_dist_need;
     int istep;
     double tt;
 };
@@ -922,6 +927,8 @@ void orc_destroy(orc *o) {
     for (int k = 0; k < 12; k++) free(o->grad[k]);
     free(o->f_lap); free(o->f); free(o->fn); free(o->du); free(o->Amu);
     free(o->ub1); free(o->pfx); free(o->pgz);
+    free(o->in124); free(o->up124); free(o->coeff124); free(o->udist124);
+    free(o->in124_user); free(o->up124_user);
     free(o);
 }
 
@@ -1023,6 +1030,9 @@ static void metric_derivative(orc *o, const double *a, int dir, int bound, doubl
         }
 }
 
+static void mirror_y(orc *o, double *a, double sign);
+static void ibc124_mirror_metrics(orc *o);
+
 int orc_load_mesh(orc *o, const double *x, const double *y, const double *z) {
     const double *src[3] = {x, y, z};
     for (int k = 0; k < 3; k++) {
@@ -1032,6 +1042,10 @@ int orc_load_mesh(orc *o, const double *x, const double *y, const double *z) {
                     o->A[k][IL(o, xx + LAP, yy + LAP, zz + LAP)] = src[k][I0(o, xx, yy, zz)];
         /* exchange_boundary_xyz on the host copy, src/OCFD_Comput_Jacobian3d.cu:139-141 */
         fill_xyz(o, o->A[k]);
+    }
+    /* boundary_Jac3d_Axx (src/OCFD_Comput_Jacobian3d.cu:592-617, IBC 124) */
+    if (o->c.ibc == 124 && o->c.bc_npara[0] == 1) {
+        mirror_y(o, o->A[AXX], 1.0); mirror_y(o, o->A[AYY], -1.0); mirror_y(o, o->A[AZZ], 1.0);
     }
     /* comput_Jac3d, src/OCFD_Comput_Jacobian3d.cu:393-424 */
     long N = o->nint;
@@ -1089,6 +1103,7 @@ int orc_load_mesh(orc *o, const double *x, const double *y, const double *z) {
         /* src/OCFD_Comput_Jacobian3d.cu:225-234 */
         fill_xyz(o, a);
     }
+    ibc124_mirror_metrics(o);
     free(D);
     return 0;
 }
@@ -1104,6 +1119,7 @@ int orc_load_jacobi(orc *o, const double *a13) {
                     o->A[k][IL(o, xx + LAP, yy + LAP, zz + LAP)] = s[I0(o, xx, yy, zz)];
         fill_xyz(o, o->A[k]);
     }
+    ibc124_mirror_metrics(o); /* src/OCFD_init.cu:116-122 */
     return 0;
 }
 
@@ -1272,9 +1288,215 @@ static void ibc108(orc *o) {
     }
 }
 
+/* y-ghost mirror about j = 0 and j = ny-1 over the interior x and z,
+ * boundary_Jac3d_kernal_y_l/r (src/OCFD_Comput_Jacobian3d.cu:429-448) and
+ * do_symmetry_kernel_m/p (src/OCFD_boundary_Liftbody3D.cu:84-115). */
+static void mirror_y(orc *o, double *a, double sign) {
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    for (int z = LAP; z < nz + LAP; z++)
+        for (int x = LAP; x < nx + LAP; x++) {
+            for (int y = 0; y < LAP; y++) a[IL(o, x, y, z)] = sign * a[IL(o, x, 2 * LAP - y, z)];
+            for (int y = ny + LAP; y < ny + 2 * LAP; y++) a[IL(o, x, y, z)] = sign * a[IL(o, x, 2 * (ny + LAP - 1) - y, z)];
+        }
+}
+
+/* boundary_Jac3d_Liftbody_Ajac (src/OCFD_Comput_Jacobian3d.cu:619-662) */
+static void ibc124_mirror_metrics(orc *o) {
+    if (o->c.ibc != 124 || o->c.bc_npara[0] != 1) return;
+    static const double sign[10] = {1, -1, 1, -1, 1, -1, 1, -1, 1, 1};
+    for (int k = AKX; k <= AJAC; k++) mirror_y(o, o->A[k], sign[k - AKX]);
+}
+
+int orc_set_bc_tables(orc *o, const double *inlet5, const double *upper5) {
+    long nyz = (long)o->ny * o->nz, nxy = (long)o->nx * o->ny;
+    if (inlet5) {
+        free(o->in124_user);
+        o->in124_user = (double *)malloc(sizeof(double) * 5 * nyz);
+        memcpy(o->in124_user, inlet5, sizeof(double) * 5 * nyz);
+    }
+    if (upper5) {
+        free(o->up124_user);
+        o->up124_user = (double *)malloc(sizeof(double) * 5 * nxy);
+        memcpy(o->up124_user, upper5, sizeof(double) * 5 * nxy);
+    }
+    return 0;
+}
+
+/* bc_user_Liftbody3d_init (src/OCFD_boundary_init.c:95-318), parameters of
+ * bc_parameter (:50-71); rand() from glibc's default seed (the reference
+ * seeds it with the clock). */
+static int ibc124_init(orc *o) {
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    long nyz = (long)ny * nz, nxy = (long)nx * ny;
+    int withleading = o->c.bc_npara[1], upper_mode = o->c.bc_npara[2];
+    int MZMAX = o->c.bc_npara[3], MTMAX = o->c.bc_npara[4];
+    double EPSL_WALL = o->c.bc_rpara[2], EPSL_UPPER = o->c.bc_rpara[3];
+    double DB = o->c.bc_rpara[5], DE = o->c.bc_rpara[6];
+    const double PI = 3.141592653589793;
+    if (!o->in124) {
+        o->in124 = zalloc(5 * nyz);
+        o->up124 = zalloc(5 * nxy);
+        o->coeff124 = zalloc(3 * nxy);
+        o->udist124 = zalloc(nxy);
+    }
+    double *d[5] = {o->d, o->u, o->v, o->w, o->T};
+    if (withleading == 1) {
+        if (!o->in124_user) return -1;
+        memcpy(o->in124, o->in124_user, sizeof(double) * 5 * nyz);
+    } else {
+        for (int m = 0; m < 5; m++)
+            for (int k = 0; k < nz; k++)
+                for (int j = 0; j < ny; j++) o->in124[m * nyz + (long)k * ny + j] = d[m][IL(o, LAP, j + LAP, k + LAP)];
+    }
+    if (upper_mode == 1) {
+        if (!o->up124_user) return -1;
+        memcpy(o->up124, o->up124_user, sizeof(double) * 5 * nxy);
+    }
+    const double *X = o->A[AXX], *Y = o->A[AYY], *Z = o->A[AZZ];
+    srand(1);
+    o->v_dist_need = 0;
+    if (MZMAX != 0) {
+        /* get_xy_blow_suction_multiwave (:320-372) */
+        o->v_dist_need = 1;
+        int MZ1 = abs(MZMAX);
+        double *faiz = zalloc(MZ1 + 1), *zl = zalloc(MZ1 + 1), *fx = zalloc(nx), *gz = zalloc(ny);
+        double ztmp = 0.;
+        for (int k = 0; k < MZ1; k++) {
+            faiz[k] = rand() / (double)RAND_MAX;
+            if (k == 0) zl[k] = 1.; else zl[k] = zl[k - 1] / 1.25;
+            ztmp = ztmp + zl[k];
+        }
+        for (int k = 0; k < MZ1; k++) zl[k] = zl[k] / ztmp;
+        for (int i = 0; i < nx; i++) {
+            double xa = X[IL(o, i + LAP, LAP, LAP)];
+            if (xa >= DB && xa <= DE) {
+                double seta = 10. * PI * (xa - DB) / (DB - DE);
+                fx[i] = 4. / sqrt(27.) * sin(seta) * (1. - cos(seta));
+            } else fx[i] = 0.;
+        }
+        for (int j = 0; j < ny; j++) {
+            gz[j] = 0.;
+            double seta = ((double)j) / ny;
+            if (MZMAX > 0) {
+                for (int m = 0; m < MZMAX; m++) gz[j] = gz[j] + zl[m] * sin(2. * PI * (m + 1) * (seta + faiz[m]));
+            } else {
+                gz[j] = sin(-2. * PI * MZMAX * seta);
+            }
+        }
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double vdw = fx[i] * gz[j];
+                long q = (long)j * nx + i;
+                o->coeff124[q] = 0;
+                o->coeff124[nxy + q] = EPSL_WALL * vdw * sin(2 * PI / ny * j);
+                o->coeff124[2 * nxy + q] = EPSL_WALL * vdw * cos(2 * PI / ny * j);
+            }
+        free(faiz); free(zl); free(fx); free(gz);
+    } else {
+        double *tv = zalloc(nxy), *tu = zalloc(nxy);
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                double r = (rand() / (double)RAND_MAX - 0.5) * 2.0;
+                tv[(long)j * nx + i] = EPSL_WALL * r;
+                r = (rand() / (double)RAND_MAX - 0.5) * 2.0;
+                tu[(long)j * nx + i] = EPSL_UPPER * r;
+            }
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                long q = (long)j * nx + i;
+                long L0 = IL(o, i + LAP, j + LAP, LAP), L1 = IL(o, i + LAP, j + LAP, LAP + 1);
+                double vdw = 0.0;
+                if (X[L0] >= DB && X[L0] <= DE) {
+                    o->v_dist_need = 1;
+                    vdw = tv[q];
+                }
+                o->udist124[q] = tu[q];
+                double xn = X[L1] - X[L0], yn = Y[L1] - Y[L0], zn = Z[L1] - Z[L0];
+                double sn = sqrt(xn * xn + yn * yn + zn * zn);
+                o->coeff124[q] = vdw * xn / sn;
+                o->coeff124[nxy + q] = vdw * yn / sn;
+                o->coeff124[2 * nxy + q] = vdw * zn / sn;
+            }
+        free(tv); free(tu);
+    }
+    /* get_fait_multifrequancy (:501-517) */
+    double Ttmp = 0.;
+    for (int k = 0; k < MTMAX && k < 64; k++) {
+        o->fait[k] = rand() / (double)RAND_MAX;
+        o->TM[0] = 1.;
+        if (k > 0) o->TM[k] = o->TM[k - 1] / 1.25;
+        Ttmp = Ttmp + o->TM[k];
+    }
+    for (int k = 0; k < MTMAX && k < 64; k++) o->TM[k] = o->TM[k] / Ttmp;
+    return 0;
+}
+
+/* bc_user_Liftbody3d (src/OCFD_boundary_Liftbody3D.cu:185-294) */
+static void ibc124(orc *o) {
+    int nx = o->nx, ny = o->ny, nz = o->nz;
+    long nyz = (long)ny * nz, nxy = (long)nx * ny;
+    double *d = o->d, *u = o->u, *v = o->v, *w = o->w, *T = o->T;
+    /* do_u2d_inlet_kernel: i = 0..LAP */
+    for (int z = LAP; z < nz + LAP; z++)
+        for (int y = LAP; y < ny + LAP; y++) {
+            long q = (long)(z - LAP) * ny + (y - LAP);
+            for (int i = 0; i <= LAP; i++) {
+                long L = IL(o, i, y, z);
+                d[L] = o->in124[q]; u[L] = o->in124[nyz + q]; v[L] = o->in124[2 * nyz + q];
+                w[L] = o->in124[3 * nyz + q]; T[L] = o->in124[4 * nyz + q];
+            }
+        }
+    /* upper face k = nz-1 */
+    double aoa = o->c.bc_rpara[0] * 3.141592653589793 / 180;
+    double sin_aoa = sin(aoa), cos_aoa = cos(aoa);
+    for (int y = LAP; y < ny + LAP; y++)
+        for (int x = LAP; x < nx + LAP; x++) {
+            long q = (long)(y - LAP) * nx + (x - LAP), L = IL(o, x, y, nz + LAP - 1);
+            double dist = o->udist124[q];
+            if (o->c.bc_npara[2] == 1) { /* do_u2d_upper_kernel */
+                d[L] = o->up124[q]; u[L] = o->up124[nxy + q] + dist; v[L] = o->up124[2 * nxy + q];
+                w[L] = o->up124[3 * nxy + q]; T[L] = o->up124[4 * nxy + q];
+            } else { /* do_u_dist_upper_kernel */
+                d[L] = 1.0; u[L] = cos_aoa + dist; v[L] = 0.0; w[L] = sin_aoa; T[L] = 1.0;
+            }
+        }
+    /* wall k = 0 */
+    double ht = 0.;
+    double BETA = o->c.bc_rpara[4];
+    int MTMAX = o->c.bc_npara[4];
+    if (BETA > 0.) {
+        for (int m = 0; m < MTMAX; m++) ht = ht + o->TM[m] * sin((m + 1) * BETA * o->tt);
+    } else ht = 1.;
+    double TW = o->c.bc_rpara[1];
+    for (int y = LAP; y < ny + LAP; y++)
+        for (int x = LAP; x < nx + LAP; x++) {
+            long q = (long)(y - LAP) * nx + (x - LAP);
+            long L0 = IL(o, x, y, LAP), L1 = IL(o, x, y, LAP + 1), L2 = IL(o, x, y, LAP + 2);
+            if (o->v_dist_need) {
+                u[L0] = o->coeff124[q] * ht;
+                v[L0] = o->coeff124[nxy + q] * ht;
+                w[L0] = o->coeff124[2 * nxy + q] * ht;
+            } else {
+                u[L0] = 0.0; v[L0] = 0.0; w[L0] = 0.0;
+            }
+            if (TW > 0.0) {
+                T[L0] = TW;
+                d[L0] = (4.0 * d[L1] * T[L1] - d[L2] * T[L2]) / (3.0 * TW);
+            } else {
+                T[L0] = (4.0 * T[L1] - T[L2]) / 3.0;
+                d[L0] = (4.0 * d[L1] - d[L2]) / 3.0;
+            }
+        }
+    /* symmetry planes */
+    if (o->c.bc_npara[0] == 1) {
+        mirror_y(o, d, 1.0); mirror_y(o, u, 1.0); mirror_y(o, v, -1.0); mirror_y(o, w, 1.0); mirror_y(o, T, 1.0);
+    }
+}
+
 /* OCFD_bc, src/OCFD_boundary.cu:18-92 */
 static void ocfd_bc(orc *o) {
     if (o->c.ibc == 108) ibc108(o);
+    if (o->c.ibc == 124) ibc124(o);
     int nx = o->nx, ny = o->ny, nz = o->nz;
     pri_to_cons(o, 0, 1, 0, ny, 0, nz);
     pri_to_cons(o, nx - 1, nx, 0, ny, 0, nz);
@@ -1320,6 +1542,7 @@ int orc_init(orc *o) {
     /* pri_to_cons over the interior, src/OCFD_init.cu:99-110 */
     pri_to_cons(o, 0, o->nx, 0, o->ny, 0, o->nz);
     if (o->c.ibc == 108) ibc108_init(o);
+    if (o->c.ibc == 124 && ibc124_init(o) != 0) return -1;
     /* NS_solver_real prologue, src/OCFD_NS_Solver.cu:46-54 */
     fill_xyz(o, o->d); fill_xyz(o, o->u); fill_xyz(o, o->v); fill_xyz(o, o->w); fill_xyz(o, o->T);
     ocfd_bc(o);
@@ -1725,6 +1948,7 @@ int orc_get_state(orc *o, double *f5) { memcpy(f5, o->f, sizeof(double) * NV * o
 int orc_get_du(orc *o, double *du5) { memcpy(du5, o->du, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_pressure(orc *o, double *p) { get_interior(o, o->P, p); return 0; }
 int orc_get_pressure_lap(orc *o, double *p) { memcpy(p, o->P, sizeof(double) * o->nlap); return 0; }
+int orc_get_metric_lap(orc *o, int k, double *a) { memcpy(a, o->A[k], sizeof(double) * o->nlap); return 0; }
 int orc_get_amu(orc *o, double *amu) { memcpy(amu, o->Amu, sizeof(double) * o->nint); return 0; }
 int orc_get_metrics(orc *o, double *a13) {
     for (int k = 0; k < 13; k++) get_interior(o, o->A[k], a13 + (long)k * o->nint);

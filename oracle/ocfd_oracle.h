@@ -31,6 +31,8 @@ int orc_load_jacobi(orc *o, const double *a13);
 int orc_load_flow(orc *o, const double *d, const double *u, const double *v,
                   const double *w, const double *T, int istep, double tt);
 int orc_set_inlet_profile(orc *o, const double *prof4xny);
+/* IBC 124 tables: inlet5 (5 x nz*ny), upper5 (5 x ny*nx); either NULL. */
+int orc_set_bc_tables(orc *o, const double *inlet5, const double *upper5);
 int orc_init(orc *o);
 int orc_rhs(orc *o, int krk);
 int orc_step(orc *o, int nsteps);
@@ -43,6 +45,8 @@ int orc_get_du(orc *o, double *du5);
 int orc_get_pressure(orc *o, double *p);
 /* The pressure on the whole LAP box ((nz+8)(ny+8)(nx+8), x fastest). */
 int orc_get_pressure_lap(orc *o, double *p);
+/* Metric array k (0..12: Axx Ayy Azz Akx ... Ajac) over the LAP box. */
+int orc_get_metric_lap(orc *o, int k, double *a);
 int orc_get_amu(orc *o, double *amu);
 int orc_get_metrics(orc *o, double *a13);
 int orc_get_flags(orc *o, signed char *fx, signed char *fy, signed char *fz);

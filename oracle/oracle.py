@@ -47,6 +47,7 @@ def lib() -> C.CDLL:
             "orc_load_jacobi": (C.c_int, [vp, dp]),
             "orc_load_flow": (C.c_int, [vp, dp, dp, dp, dp, dp, C.c_int, C.c_double]),
             "orc_set_inlet_profile": (C.c_int, [vp, dp]),
+            "orc_set_bc_tables": (C.c_int, [vp, C.c_void_p, C.c_void_p]),
             "orc_init": (C.c_int, [vp]),
             "orc_rhs": (C.c_int, [vp, C.c_int]),
             "orc_step": (C.c_int, [vp, C.c_int]),
@@ -56,6 +57,7 @@ def lib() -> C.CDLL:
             "orc_get_du": (C.c_int, [vp, dp]),
             "orc_get_pressure": (C.c_int, [vp, dp]),
             "orc_get_pressure_lap": (C.c_int, [vp, dp]),
+            "orc_get_metric_lap": (C.c_int, [vp, C.c_int, dp]),
             "orc_get_amu": (C.c_int, [vp, dp]),
             "orc_get_metrics": (C.c_int, [vp, dp]),
             "orc_get_flags": (C.c_int, [vp, i8, i8, i8]),
@@ -120,6 +122,12 @@ class Oracle:
     def set_inlet_profile(self, prof):
         self.L.orc_set_inlet_profile(self.h, self._flat(prof))
 
+    def set_bc_tables(self, inlet5=None, upper5=None):
+        a = None if inlet5 is None else self._flat(inlet5)
+        b = None if upper5 is None else self._flat(upper5)
+        self.L.orc_set_bc_tables(self.h, None if a is None else a.ctypes.data_as(C.c_void_p),
+                                 None if b is None else b.ctypes.data_as(C.c_void_p))
+
     def init(self):
         self.L.orc_init(self.h)
 
@@ -159,6 +167,12 @@ class Oracle:
         out = out.reshape(self.nz + 8, self.ny + 8, self.nx + 8)
         c = 4 - ghosts
         return out[c:out.shape[0] - c, c:out.shape[1] - c, c:out.shape[2] - c]
+
+    def get_metric_lap(self, k):
+        """Metric array k (0..12) over the whole LAP box (nz+8, ny+8, nx+8)."""
+        out = np.empty((self.nx + 8) * (self.ny + 8) * (self.nz + 8))
+        self.L.orc_get_metric_lap(self.h, int(k), out)
+        return out.reshape(self.nz + 8, self.ny + 8, self.nx + 8)
 
     def get_amu(self):
         out = np.empty(self.nx * self.ny * self.nz)

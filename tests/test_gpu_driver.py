@@ -48,3 +48,26 @@ def test_driver_ramp_with_analysis_events(tmp_path):
     avg, na, ta = capi.read_flow_file(tmp_path / f"OCFD{nsteps:08d}.average", 48, 32, 10)
     assert na == nsteps
     assert os.path.getsize(tmp_path / "Savedata-XY000.dat") == 4 + 12 + 4 + 5 * (4 + 48 * 32 * 8 + 4)
+
+
+def test_driver_liftbody_reads_boundary_tables(tmp_path):
+    """IBC 124 through the driver: flow-inlet-section.dat and
+    flow-outboundary.dat read by init, symmetry planes, the adiabatic wall with
+    the multi-wave disturbance."""
+    nsteps = 3
+    case = cases.liftbody(40, 20, 16, nsteps=nsteps, mzmax=3, tw=-1.0, upper=1, withleading=1)
+    case.cfg.kstep_save = nsteps
+    case.write(tmp_path)
+    print(f"[driver] liftbody 40x20x16, {nsteps} steps, in {tmp_path}", flush=True)
+    r = subprocess.run([str(EXE)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    sys.stderr.write(r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "read inlet boundary data" in r.stdout and "read upper boundary data" in r.stdout
+    prims, istep, tt = capi.read_flow_file(tmp_path / f"OCFD{nsteps:08d}.dat", 40, 20, 16)
+    o = case.load(O.Oracle(case.cfg))
+    o.step(nsteps)
+    po, io, to = o.get_flow()
+    e = [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(prims, po)]
+    print("[driver] liftbody flow file vs oracle: " + " ".join(f"{x:.2e}" for x in e), flush=True)
+    assert istep == io == nsteps and max(e) <= 1e-12

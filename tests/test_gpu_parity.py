@@ -64,6 +64,24 @@ def test_rhs_closures_low_high_both(scheme, bounds, nonref):
     assert (e <= TOL).all(), e
 
 
+@pytest.mark.parametrize("kw", [{}, {"mzmax": 3, "tw": -1.0, "upper": 1, "withleading": 1},
+                                {"mzmax": -2, "scheme": "SCHEME_HYBRIDAUTO"}])
+def test_liftbody_ibc124_steps(kw):
+    """IBC 124 on half a body of revolution: symmetry planes (metrics and
+    primitives), inlet from the flow or a table, free-stream or tabulated
+    upper face with its disturbance, isothermal or adiabatic wall with the
+    seeded or multi-wave disturbance, JAC_BOUND 0 across the symmetry
+    planes."""
+    case = cases.liftbody(40, 20, 16, nsteps=5, **kw)
+    g, o = pair(case)
+    assert np.allclose(g.get_metrics()[3:], o.get_metrics()[3:], rtol=1e-13, atol=1e-13)
+    g.step(5)
+    o.step(5)
+    e = relerr(g.get_state(), o.get_state())
+    report(f"5 steps liftbody {kw}", e)
+    assert (e <= TOL).all(), e
+
+
 @pytest.mark.parametrize("scheme", ["WENO7", "SCHEME_HYBRIDAUTO", "OMP6_HR", "WENO7_SYMBO"])
 def test_steps_wavy_box(scheme):
     case = cases.wavy_box(20, 16, 12, scheme=scheme)

@@ -378,6 +378,41 @@ def test_char_line_is_physical_line_on_linear_fluxes(scheme, side, dirn, bounds,
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
+def test_symmetry_plane_metrics_match_a_mirrored_mesh():
+    """IBC 124 symmetry planes: the metric ghosts the mirror rule writes
+    (y odd; the y row of the cofactors odd) and the metrics of the points
+    next to the planes (which read the mirrored coordinate ghosts) equal the
+    metrics of a mesh extended by the real mirror image of the grid."""
+    nx, ny, nz = 10, 12, 9
+    c = cases.liftbody(nx, ny, nz)
+    x, y, z = c.mesh
+    cfg = c.cfg
+    o = O.Oracle(cfg)
+    o.load_mesh(x, y, z)
+    half = [o.get_metric_lap(k) for k in range(13)]
+    # the grid plus its mirror images, 8 rows beyond each symmetry plane (so
+    # that the rows compared are 4 or more from the extended grid's edges)
+    ext = [np.concatenate([a[:, 8:0:-1, :] * s_, a, a[:, -2:-10:-1, :] * s_], axis=1)
+           for a, s_ in ((x, 1.0), (y, -1.0), (z, 1.0))]
+    cfg2 = capi.make_config(nx=nx, ny=ny + 16, nz=nz, iperiodic=(0, 0, 0), jac_bound=(1, 1, 1), re=100.0, ama=0.5,
+                            dt=1e-3)
+    o2 = O.Oracle(cfg2)
+    o2.load_mesh(*ext)
+    full = [o2.get_metric_lap(k) for k in range(13)]
+    # the extended grid has more points in y: h_eta = 1/(ny+15) instead of
+    # 1/(ny-1), so its cofactors with an eta derivative (Ak*, As*) and its
+    # 1/Jacobian scale by the ratio
+    r = (ny + 15) / (ny - 1)
+    scale = {3: 1 / r, 4: 1 / r, 5: 1 / r, 6: 1.0, 7: 1.0, 8: 1.0, 9: 1 / r, 10: 1 / r, 11: 1 / r, 12: r}
+    # interior x and z (what the rule covers), y rows -4..ny+3 of the half grid
+    # = rows 4..ny+11 of the extended one (LAP-box rows 0..ny+7 and 8..ny+15)
+    xs, zs = slice(4, nx + 4), slice(4, nz + 4)
+    for k in range(3, 13):
+        a = half[k][zs, 0:ny + 8, xs]
+        b = full[k][zs, 8:ny + 16, xs] * scale[k]
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max()), k
+
+
 def test_sutherland_at_reference_temperature():
     c = cases.tgv(8, nsteps=1)
     n = 8
