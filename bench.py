@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--sensor-params", default=str(REPO / "opencfd-scu_amd" / "pyocfd" / "sensor_params.npy"),
                     help="learned-sensor weights for the mixed + learned-sensor leg (skipped if absent)")
     ap.add_argument("--cpu-sample-steps", type=int, default=15)
+    ap.add_argument("--profile-steps", type=int, default=10, help="steps of the per-kernel timing pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -173,17 +174,23 @@ def main():
     if args.warmup:
         solver.step(args.warmup)
     solver.sync()
-    solver.profile(True)
-    solver.profile_reset()
     t0 = time.perf_counter()
     solver.step(args.steps)
     solver.sync()
     wall = time.perf_counter() - t0
     ev_ms = solver.last_step_ms()
-    prof = solver.profile_entries()
-    solver.profile(False)
     secs = ev_ms * 1e-3 if ev_ms > 0 else wall
     value = npts * 3 * args.steps / secs
+    # per-kernel times from a few more steps with every launch bracketed by
+    # events (this runs the kernels in order on one stream, without the
+    # overlap of the viscous fluxes with the sweeps that the timed steps use)
+    log("timing the kernels")
+    solver.profile(True)
+    solver.profile_reset()
+    solver.step(args.profile_steps)
+    solver.sync()
+    prof = solver.profile_entries()
+    solver.profile(False)
 
     # dominant kernel of the timed region
     kname, (launches, tot_ms) = max(prof.items(), key=lambda kv: kv[1][1])

@@ -48,6 +48,10 @@ struct ocfd_ctx {
     ocfd_config cfg;
     int device = 0;
     hipStream_t st = nullptr;
+    // second stream: the viscous gradients/fluxes run beside the inviscid
+    // sweeps (they share inputs only); joined before the divergence
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Geo g;
     Consts c;
     long long fs = 0;  // elements per field
@@ -383,6 +387,27 @@ int rhs(ocfd_ctx* x, int krk) {
         }
     }
     TRY(fill_prims<S>(x));
+    ViscArgsT<S> va;
+    va.g = x->g;
+    va.c = x->c;
+    va.svis = C.scheme_vis;
+    for (int q = 0; q < 6; q++) va.bnd[q] = C.dif_bound[q];
+    va.u = V.u; va.v = V.v; va.w = V.w; va.T = V.T; va.amu = V.amu;
+    for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
+    va.ev = V.ev; va.du = V.du; va.fs = x->fs;
+    // the viscous gradients/fluxes read the primitives and metrics and write
+    // only Ev: on the second stream they overlap the sweeps (which write only
+    // du).  Kernel timing runs everything in order on one stream.
+    const bool overlap = !x->prof;
+    if (overlap) {
+        HIPCK(hipEventRecord(x->ev_fork, x->st));
+        HIPCK(hipStreamWaitEvent(x->st2, x->ev_fork, 0));
+        launch_visc_gf(va, x->st2);
+        for (int dir = 0; dir < 3; dir++)
+            if (C.iperiodic[dir]) launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st2);
+        HIPCK(hipGetLastError());
+        HIPCK(hipEventRecord(x->ev_join, x->st2));
+    }
     static const char* names[3] = {"sweep_x", "sweep_y", "sweep_z"};
     const int mk[3][3] = {{M_KX, M_KY, M_KZ}, {M_IX, M_IY, M_IZ}, {M_SX, M_SY, M_SZ}};
     for (int dir = 0; dir < 3; dir++) {
@@ -405,18 +430,14 @@ int rhs(ocfd_ctx* x, int krk) {
             TRY(timed(x, names[dir], [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
         if (lrc) return fail(OCFD_ERR_UNSUPPORTED, "scheme not supported");
     }
-    ViscArgsT<S> va;
-    va.g = x->g;
-    va.c = x->c;
-    va.svis = C.scheme_vis;
-    for (int q = 0; q < 6; q++) va.bnd[q] = C.dif_bound[q];
-    va.u = V.u; va.v = V.v; va.w = V.w; va.T = V.T; va.amu = V.amu;
-    for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-    va.ev = V.ev; va.du = V.du; va.fs = x->fs;
-    TRY(timed(x, "visc_flux", [&] { launch_visc_gf(va, x->st); }));
-    for (int dir = 0; dir < 3; dir++)
-        if (C.iperiodic[dir])
-            TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
+    if (overlap) {
+        HIPCK(hipStreamWaitEvent(x->st, x->ev_join, 0));
+    } else {
+        TRY(timed(x, "visc_flux", [&] { launch_visc_gf(va, x->st); }));
+        for (int dir = 0; dir < 3; dir++)
+            if (C.iperiodic[dir])
+                TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
+    }
     for (int c0 = 0; c0 < 4; c0 += visc_div_components<S>())
         TRY(timed(x, "visc_div", [&] { launch_visc_div(va, c0, x->st); }));
     return 0;
@@ -788,9 +809,12 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
         delete x;
         return fail(OCFD_ERR_HIP, "hipSetDevice failed");
     }
-    if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
-        delete x;
-        return fail(OCFD_ERR_HIP, "hipStreamCreate failed");
+    if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&x->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&x->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&x->ev_join, hipEventDisableTiming) != hipSuccess) {
+        ocfd_destroy(x);
+        return fail(OCFD_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
     Geo& g = x->g;
     g.nx = C.nx; g.ny = C.ny; g.nz = C.nz;
@@ -897,7 +921,11 @@ int ocfd_destroy(ocfd_ctx* x) {
     for (auto e : x->pool) (void)hipEventDestroy(e);
     if (x->step_a) (void)hipEventDestroy(x->step_a);
     if (x->step_b) (void)hipEventDestroy(x->step_b);
+    if (x->st2) (void)hipStreamSynchronize(x->st2);
     for (void* p : x->allocs) (void)hipFree(p);
+    if (x->ev_fork) (void)hipEventDestroy(x->ev_fork);
+    if (x->ev_join) (void)hipEventDestroy(x->ev_join);
+    if (x->st2) (void)hipStreamDestroy(x->st2);
     if (x->st) (void)hipStreamDestroy(x->st);
     delete x;
     return 0;
