@@ -52,6 +52,7 @@ struct ocfd_ctx {
     // sweeps (they share inputs only); joined before the divergence
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool serial = false;  // OCFD_SERIAL=1: one stream (per-kernel timing with an external profiler)
     Geo g;
     Consts c;
     long long fs = 0;  // elements per field
@@ -398,7 +399,7 @@ int rhs(ocfd_ctx* x, int krk) {
     // the viscous gradients/fluxes read the primitives and metrics and write
     // only Ev: on the second stream they overlap the sweeps (which write only
     // du).  Kernel timing runs everything in order on one stream.
-    const bool overlap = !x->prof;
+    const bool overlap = !x->prof && !x->serial;
     if (overlap) {
         HIPCK(hipEventRecord(x->ev_fork, x->st));
         HIPCK(hipStreamWaitEvent(x->st2, x->ev_fork, 0));
@@ -816,6 +817,7 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
         ocfd_destroy(x);
         return fail(OCFD_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
+    if (const char* e = std::getenv("OCFD_SERIAL")) x->serial = std::atoi(e) != 0;
     Geo& g = x->g;
     g.nx = C.nx; g.ny = C.ny; g.nz = C.nz;
     // row pitch: 128-byte rows in FP64; 32 elements (128 bytes) in FP32 too
