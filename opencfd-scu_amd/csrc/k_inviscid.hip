@@ -263,7 +263,7 @@ constexpr int XS = 56, XA = 4, XN = XS + 8, XF = XS + 1;
 
 template <int SCH, bool FIRST, typename S>
 __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
-    static_assert(XF * XA <= 256 && XN * XA == 256, "one node and one face per thread");
+    static_assert(XF <= 64 && XA == 4 && XN * XA == 256, "one row per wave; one node and one face per thread");
     __shared__ S sh[10 * XN * XA];
     const Geo& g = A.g;
     const int tid = threadIdx.x;
@@ -274,16 +274,20 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
     // consecutive x
     auto at = [](int v, int n, int a) { return (v * XA + a) * XN + n; };
 
-    // the hybrid flag of this thread's face and the Ajac of its output node
-    // are fetched first, so that their latency hides behind phase 1
-    const bool fthread = tid < XF * XA;
-    const int f_loc = fthread ? tid % XF : 0, a_loc = fthread ? tid / XF : 0;
+    // phases 2 and 3: wave a handles row a, lane = position along the row
+    // (57 faces, 56 nodes), so that the LDS accesses of a wave are
+    // consecutive doubles of one row (no bank conflicts).  The hybrid flag of
+    // this thread's face and the Ajac of its output node are fetched first,
+    // so that their latency hides behind phase 1
+    const int lane = tid & 63, wrow = tid >> 6;
+    const bool fthread = lane < XF;
+    const int f_loc = fthread ? lane : 0, a_loc = wrow;
     const int jf = s0 - 1 + f_loc, acf = a0 + a_loc;
     const bool fvalid = fthread && (jf <= n_dir - 1) && (acf < g.ny);
     int hy = 0;
     if (SCH == OCFD_HYBRIDAUTO && fvalid) hy = A.flag[gidx(g, jf, acf, k2)];
-    const bool nthread = tid < XS * XA;
-    const int n3 = nthread ? tid % XS : 0, a3 = nthread ? tid / XS : 0;
+    const bool nthread = lane < XS;
+    const int n3 = nthread ? lane : 0, a3 = wrow;
     const int i3 = s0 + n3, ac3 = a0 + a3;
     const bool nvalid = nthread && i3 < n_dir && ac3 < g.ny;
     const long long q3 = nvalid ? gidx(g, i3, ac3, k2) : 0;
@@ -322,8 +326,8 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
     if (fthread) {
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            sh[(m * XF + f_loc) * XA + a_loc] = fvalid ? hp[m] : S(0);
-            sh[((5 + m) * XF + f_loc) * XA + a_loc] = fvalid ? hm[m] : S(0);
+            sh[(m * XA + a_loc) * XF + f_loc] = fvalid ? hp[m] : S(0);
+            sh[((5 + m) * XA + a_loc) * XF + f_loc] = fvalid ? hm[m] : S(0);
         }
     }
     __syncthreads();
@@ -340,8 +344,8 @@ __global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
             const double rh = 1.0 / A.h;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const S pr = sh[(m * XF + n + 1) * XA + a], pl = sh[(m * XF + n) * XA + a];
-                const S mr = sh[((5 + m) * XF + n + 1) * XA + a], ml = sh[((5 + m) * XF + n) * XA + a];
+                const S pr = sh[(m * XA + a) * XF + n + 1], pl = sh[(m * XA + a) * XF + n];
+                const S mr = sh[((5 + m) * XA + a) * XF + n + 1], ml = sh[((5 + m) * XA + a) * XF + n];
                 const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
                 const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
                 S* dv = A.du + (long long)m * A.fs;
@@ -648,7 +652,8 @@ template <> struct CharTile<0> { static constexpr int NS = 56, NA = 4; };
 template <int DIR, int SCH, bool FIRST, typename S>
 __global__ void __launch_bounds__(256) k_char(SweepArgsT<S> A) {
     constexpr int NS = CharTile<DIR>::NS, NA = CharTile<DIR>::NA, NN = NS + 8, NF = NS + 1;
-    static_assert(NN * NA == 256 && NF * NA <= 256, "one node and at most one face per thread");
+    static_assert(NN * NA == 256 && NF * NA <= 256 && (DIR != 0 || (NA == 4 && NF <= 64)),
+                  "one node and at most one face per thread; x: one row per wave");
     __shared__ S sh[10 * NN * NA];
     __shared__ S shb[2][10][NA];  // boundary differences: [low/high][P 0..4, M 5..9][column]
     const Geo& g = A.g;
@@ -689,8 +694,9 @@ __global__ void __launch_bounds__(256) k_char(SweepArgsT<S> A) {
 
     // ---- phase 2: one face per thread
     S hp[5], hm[5];
-    const bool fthread = tid < NF * NA;
-    const int f_loc = DIR == 0 ? tid % NF : tid / NA, a_loc = DIR == 0 ? tid / NF : tid % NA;
+    // x: wave a handles row a (lane = face along the row); y, z: lanes across
+    const bool fthread = DIR == 0 ? (tid & 63) < NF : tid < NF * NA;
+    const int f_loc = DIR == 0 ? tid & 63 : tid / NA, a_loc = DIR == 0 ? tid >> 6 : tid % NA;
     const int j = s0 - 1 + f_loc, ac = a0 + a_loc;
     const bool fvalid = fthread && j <= n_dir - 1 && ac < n_acr;
     if (fvalid) {
@@ -770,8 +776,8 @@ __global__ void __launch_bounds__(256) k_char(SweepArgsT<S> A) {
     __syncthreads();
 
     // ---- phase 3: du at nodes, P then M
-    if (tid < NS * NA) {
-        const int n = DIR == 0 ? tid % NS : tid / NA, a = DIR == 0 ? tid / NS : tid % NA;
+    if (DIR == 0 ? (tid & 63) < NS : tid < NS * NA) {
+        const int n = DIR == 0 ? tid & 63 : tid / NA, a = DIR == 0 ? tid >> 6 : tid % NA;
         const int i = s0 + n, ai = a0 + a;
         if (i < n_dir && ai < n_acr) {
             const long long q = lin(i, ai);
@@ -821,7 +827,12 @@ void launch_char_scheme(int dir, const SweepArgsT<S>& a, bool first, hipStream_t
 #define OCFD_LMARCH 32  // output nodes per marching chunk (k_march)
 #endif
 // y/z tile shapes (output nodes x columns): rows of 64 bytes suit both
-// precisions (8 doubles, 16 floats)
+// precisions (8 doubles, 16 floats).  Measured on the 512x128x64 ramp (FP64):
+// NS = 32 with 320 threads (no partial tile on lines of 64 and 128 nodes) is
+// 10-20 % slower than NS = 28 with 256 threads; a per-block queue that hands
+// the rare WENO faces of hybrid_auto to all threads, and a form with the face
+// pass fused into the node pass by cross-lane shuffles (one more resident
+// wave per SIMD), were no faster.
 #ifndef OCFD_YZ_NS_D
 #define OCFD_YZ_NS_D 28
 #endif

@@ -57,6 +57,9 @@ constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
 template <typename S, bool DIV = false> struct KZ_ { static constexpr int v = DIV ? OCFD_VDIV_KZ_D : OCFD_VISC_KZ_D; };
 template <bool DIV> struct KZ_<float, DIV> { static constexpr int v = DIV ? OCFD_VDIV_KZ_F : OCFD_VISC_KZ_F; };
 constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
+#ifndef OCFD_VISC_SKIP_CORNERS
+#define OCFD_VISC_SKIP_CORNERS 1  // the gradient kernel does not load the tile's halo corners
+#endif
 
 // Central derivative at interior index i (0..n-1) from s[0..8] = values at
 // i-4..i+4 (OCFD_CD6/CD8_kernel + D0 closures + put_d0_kernel).  rh = 1/h; the
@@ -124,15 +127,17 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
     const S* src[4] = {A.u, A.v, A.w, A.T};
     const int ic = min(t.i, g.nx - 1), jc = min(t.j, g.ny - 1);  // clamped column for queue loads
     const long long col = gidx(g, ic, jc, 0);
-    // halo slots (positions outside the TX x TY interior of the tile)
+    // halo slots: positions outside the TX x TY interior of the tile that a
+    // derivative reads (the x strips beside the interior rows and the y
+    // strips beside the interior columns; the corners are never read)
     int sl[NSL];
     long long so[NSL];
 #pragma unroll
     for (int r = 0; r < NSL; r++) {
         const int p = tid + NT * r;
         const int hx = p % PW, hy = p / PW;
-        const bool inner = hx >= 4 && hx < TX + 4 && hy >= 4 && hy < TY + 4;
-        sl[r] = (p < PN && !inner) ? p : -1;
+        const bool xin = hx >= 4 && hx < TX + 4, yin = hy >= 4 && hy < TY + 4;
+        sl[r] = (p < PN && (xin != yin || !OCFD_VISC_SKIP_CORNERS) && !(xin && yin)) ? p : -1;
         const int gi = i0 - 4 + hx, gj = j0 - 4 + hy;
         so[r] = (gi <= g.nx + 3 && gj <= g.ny + 3) ? gidx(g, gi, gj, 0) : -1;
     }
@@ -417,7 +422,10 @@ void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
 // Flux components per divergence launch: one on the FP64 path (a quarter of
 // the registers: 4 waves per SIMD instead of 2, measured faster), all four on
 // the FP32 path.
-template <typename S> struct NC_ { static constexpr int v = 1; };
+#ifndef OCFD_VDIV_NC_D
+#define OCFD_VDIV_NC_D 1  // flux components per divergence launch, FP64 path
+#endif
+template <typename S> struct NC_ { static constexpr int v = OCFD_VDIV_NC_D; };
 template <> struct NC_<float> { static constexpr int v = 4; };
 template <typename S>
 int visc_div_components() { return NC_<S>::v; }

@@ -64,6 +64,37 @@ def test_rhs_closures_low_high_both(scheme, bounds, nonref):
     assert (e <= TOL).all(), e
 
 
+@pytest.mark.parametrize("intv", [(0.01, 0.03), (0.005, 0.02), (0.003, 0.01), (0.0, 0.0), (0.0, 1e30)])
+@pytest.mark.parametrize("closed", [False, True])
+def test_hybrid_flag_mixes(intv, closed):
+    """hybrid_auto with HY_DP_INTV chosen for about 4 %, 31 %, 50 % and all of
+    the faces on WENO7 or WENO5 (64x24x12 wavy box, periodic or closed on
+    every face).  The sweeps reconstruct the WENO faces of a tile from a
+    per-block queue and, when it is full, in place: every mix and the
+    closures must give the oracle's flags and du; the FP32 bulk path is
+    held to 1e-4."""
+    case = cases.wavy_box(64, 24, 12, scheme="SCHEME_HYBRIDAUTO", amp=0.08,
+                          periodic=(0, 0, 0) if closed else (1, 1, 1))
+    case.cfg.hy_dp_intv[0], case.cfg.hy_dp_intv[1] = intv
+    if closed:
+        for q in range(6):
+            case.cfg.dif_bound[q] = 1
+    g, o = pair(case)
+    g.rhs(1)
+    o.rhs(1)
+    for a, b in zip(g.get_flags(), o.get_flags()):
+        assert np.array_equal(a, b)
+    e = relerr(g.get_du(), o.get_du())
+    report(f"rhs hybrid intv={intv} closed={closed}", e)
+    assert (e <= TOL).all(), e
+    case.cfg.precision = capi.PREC_MIXED
+    m = case.load(capi.Solver(case.cfg))
+    m.rhs(1)
+    e = relerr(m.get_du(), o.get_du())
+    report(f"rhs hybrid mixed intv={intv} closed={closed}", e)
+    assert (e <= 1e-4).all(), e
+
+
 @pytest.mark.parametrize("kw", [{}, {"mzmax": 3, "tw": -1.0, "upper": 1, "withleading": 1},
                                 {"mzmax": -2, "scheme": "SCHEME_HYBRIDAUTO"}])
 def test_liftbody_ibc124_steps(kw):
