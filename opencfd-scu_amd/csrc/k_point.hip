@@ -45,9 +45,14 @@ __global__ void __launch_bounds__(256) k_stage(StageArgsT<S> A) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
     if (i >= A.g.nx || j >= A.g.ny) return;
     const long long q = gidx(A.g, i, j, k);
+    // du, plus its viscous part dv (components 1..4) when that is kept apart
     double du[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) du[m] = double(A.du[m * A.fs + q]);
+    if (A.dv) {
+#pragma unroll
+        for (int m = 1; m < 5; m++) du[m] += double(A.dv[(m - 1) * A.fs + q]);
+    }
     stage_point(A, q, du);
 }
 

@@ -12,12 +12,15 @@
 //                gradients never leave registers.
 //   k_visc_div : du[1..4] += Ajac*d(Ev)/dxi, then eta, then zeta (OCFD_d?0 of
 //                Ev + YF_Pe_XF, src/OCFD_NS_Jacobian3d.cu:304-440,
-//                src/commen_kernel.cu:158-169), no atomics.
+//                src/commen_kernel.cu:158-169), no atomics; on the FP64 path
+//                the three terms go to their own buffer dv instead and the
+//                RK stage adds du + dv, so that the divergence need not wait
+//                for the inviscid sweeps.
 //   k_qcrit    : the Q criterion of the analysis events (same derivatives).
 //
 // S (double or float) is the type of u,v,w,T, mu, the derivatives, the
-// fluxes and du; the Jacobian factor of the divergence and its accumulation
-// into du are double.
+// fluxes, du and dv; the Jacobian factor of the divergence and its sums are
+// double.
 //
 // Both use 2.5-D blocking: a block owns a TX x TY tile of (x,y) columns and
 // marches through a chunk of z planes (gradients: 32 on the FP64 path, 8 on
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
 // One launch handles NC of the four flux components, starting at c0 (each
 // component is independent; fewer components per launch need fewer
 // registers).
-template <int SV, typename S, int NC>
+template <int SV, typename S, int NC, bool SEP>
 __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     constexpr int XW = TX + 8, XN = TY * XW, YN = (TY + 8) * TX;
     constexpr int NSX = (XN + NT - 1) / NT, NSY = (YN + NT - 1) / NT;
@@ -345,13 +348,20 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
                 for (int r = 0; r < 9; r++) s[r] = ey[m][(ty + r) * TX + tx];
                 const S vy = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
                 const S vz = cd9<SV>(q[m], k, g.nz, A.bnd[4], A.bnd[5], rhz);
-                // the Jacobian factor and the accumulation into du are double
-                S* dv = A.du + (long long)(c0 + m + 1) * fs + qi;
-                double acc = double(*dv);
-                acc = acc + ajac * double(vx);
-                acc = acc + ajac * double(vy);
-                acc = acc + ajac * double(vz);
-                *dv = S(acc);
+                // the Jacobian factor and the sums are double
+                if (SEP) {
+                    double acc = ajac * double(vx);
+                    acc = acc + ajac * double(vy);
+                    acc = acc + ajac * double(vz);
+                    A.dv[(long long)(c0 + m) * fs + qi] = S(acc);
+                } else {
+                    S* du = A.du + (long long)(c0 + m + 1) * fs + qi;
+                    double acc = double(*du);
+                    acc = acc + ajac * double(vx);
+                    acc = acc + ajac * double(vy);
+                    acc = acc + ajac * double(vz);
+                    *du = S(acc);
+                }
             }
         }
 #pragma unroll
@@ -432,8 +442,15 @@ int visc_div_components() { return NC_<S>::v; }
 template <typename S>
 void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st) {
     constexpr int NC = NC_<S>::v;
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC>), (grid25<S, true>(a.g)), dim3(TX, TY), 0, st, a, c0);
-    else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC>), (grid25<S, true>(a.g)), dim3(TX, TY), 0, st, a, c0);
+    // SEP: the terms go to dv (set) or are added to du (null)
+    const dim3 grid = grid25<S, true>(a.g), blk(TX, TY);
+    if (a.svis == OCFD_CD6) {
+        if (a.dv) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC, true>), grid, blk, 0, st, a, c0);
+        else hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC, false>), grid, blk, 0, st, a, c0);
+    } else {
+        if (a.dv) hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC, true>), grid, blk, 0, st, a, c0);
+        else hipLaunchKernelGGL((k_visc_div<OCFD_CD8, S, NC, false>), grid, blk, 0, st, a, c0);
+    }
 }
 template <typename S>
 void launch_qcrit(const ViscArgsT<S>& a, double* q, hipStream_t st) {

@@ -48,8 +48,9 @@ struct ocfd_ctx {
     ocfd_config cfg;
     int device = 0;
     hipStream_t st = nullptr;
-    // second stream: the viscous gradients/fluxes run beside the inviscid
-    // sweeps (they share inputs only); joined before the divergence
+    // second stream: the viscous gradients, fluxes and divergence run beside
+    // the inviscid sweeps (they share inputs only; the divergence writes its
+    // own buffer dv); joined before the RK stage
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool serial = false;  // OCFD_SERIAL=1: one stream (per-kernel timing with an external profiler)
@@ -62,9 +63,13 @@ struct ocfd_ctx {
     // the plain path; float (the s* pointers) on the mixed path
     bool mixed = false;
     double *d = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *T = nullptr, *P = nullptr;
-    double *du = nullptr, *amu = nullptr, *ev = nullptr;
+    // dv: the viscous part of du[1..4] on the FP64 path (the mixed path adds
+    // it to du after the sweeps: measured faster there, while on the FP64
+    // path running the divergence beside the sweeps and adding du + dv in
+    // the stage is faster)
+    double *du = nullptr, *amu = nullptr, *ev = nullptr, *dv = nullptr;
     float *sd = nullptr, *su = nullptr, *sv = nullptr, *sw = nullptr, *sT = nullptr, *sP = nullptr;
-    float *sdu = nullptr, *samu = nullptr, *sev = nullptr;
+    float *sdu = nullptr, *samu = nullptr, *sev = nullptr, *sdv = nullptr;
     // always double: metrics, conserved state, filter input, scratch
     double* M[M_COUNT] = {};
     double *f = nullptr, *fn = nullptr, *flap = nullptr;
@@ -109,14 +114,14 @@ namespace {
 // The arrays whose type follows the precision of the context.
 template <typename S>
 struct View {
-    S *d, *u, *v, *w, *T, *P, *amu, *du, *ev;
+    S *d, *u, *v, *w, *T, *P, *amu, *du, *ev, *dv;
 };
 template <typename S> View<S> view(ocfd_ctx* x);
 template <> View<double> view<double>(ocfd_ctx* x) {
-    return {x->d, x->u, x->v, x->w, x->T, x->P, x->amu, x->du, x->ev};
+    return {x->d, x->u, x->v, x->w, x->T, x->P, x->amu, x->du, x->ev, x->dv};
 }
 template <> View<float> view<float>(ocfd_ctx* x) {
-    return {x->sd, x->su, x->sv, x->sw, x->sT, x->sP, x->samu, x->sdu, x->sev};
+    return {x->sd, x->su, x->sv, x->sw, x->sT, x->sP, x->samu, x->sdu, x->sev, x->sdv};
 }
 
 hipEvent_t get_event(ocfd_ctx* x) {
@@ -395,10 +400,11 @@ int rhs(ocfd_ctx* x, int krk) {
     for (int q = 0; q < 6; q++) va.bnd[q] = C.dif_bound[q];
     va.u = V.u; va.v = V.v; va.w = V.w; va.T = V.T; va.amu = V.amu;
     for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-    va.ev = V.ev; va.du = V.du; va.fs = x->fs;
-    // the viscous gradients/fluxes read the primitives and metrics and write
-    // only Ev: on the second stream they overlap the sweeps (which write only
-    // du).  Kernel timing runs everything in order on one stream.
+    va.ev = V.ev; va.du = V.du; va.dv = V.dv; va.fs = x->fs;
+    // the viscous terms read the primitives and metrics and write only Ev
+    // and dv: on the second stream they overlap the sweeps (which write only
+    // du); the stage adds du + dv.  Kernel timing runs everything in order on
+    // one stream.
     const bool overlap = !x->prof && !x->serial;
     if (overlap) {
         HIPCK(hipEventRecord(x->ev_fork, x->st));
@@ -406,6 +412,8 @@ int rhs(ocfd_ctx* x, int krk) {
         launch_visc_gf(va, x->st2);
         for (int dir = 0; dir < 3; dir++)
             if (C.iperiodic[dir]) launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st2);
+        if (va.dv)
+            for (int c0 = 0; c0 < 4; c0 += visc_div_components<S>()) launch_visc_div(va, c0, x->st2);
         HIPCK(hipGetLastError());
         HIPCK(hipEventRecord(x->ev_join, x->st2));
     }
@@ -439,8 +447,9 @@ int rhs(ocfd_ctx* x, int krk) {
             if (C.iperiodic[dir])
                 TRY(timed(x, "fill_ev", [&] { launch_fill(x->g, V.ev + 4LL * dir * x->fs, 4, x->fs, dir, x->st); }));
     }
-    for (int c0 = 0; c0 < 4; c0 += visc_div_components<S>())
-        TRY(timed(x, "visc_div", [&] { launch_visc_div(va, c0, x->st); }));
+    if (!overlap || !va.dv)
+        for (int c0 = 0; c0 < 4; c0 += visc_div_components<S>())
+            TRY(timed(x, "visc_div", [&] { launch_visc_div(va, c0, x->st); }));
     return 0;
 }
 
@@ -499,7 +508,7 @@ StageArgsT<S> stage_args(ocfd_ctx* x, int krk, bool write_flap) {
     const View<S> V = view<S>(x);
     StageArgsT<S> sa;
     sa.g = x->g; sa.c = x->c; sa.krk = krk; sa.dt = x->cfg.dt;
-    sa.f = x->f; sa.fn = x->fn; sa.du = V.du;
+    sa.f = x->f; sa.fn = x->fn; sa.du = V.du; sa.dv = V.dv;
     sa.flap = write_flap ? x->flap : nullptr;
     sa.d = V.d; sa.u = V.u; sa.v = V.v; sa.w = V.w; sa.T = V.T; sa.P = V.P; sa.amu = V.amu;
     sa.fs = x->fs;
@@ -865,10 +874,11 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
         double* dua = rc ? nullptr : alloc_fields(x, 5, rc);
         double* am = rc ? nullptr : alloc_fields(x, 1, rc);
         double* ev = rc ? nullptr : alloc_fields(x, 12, rc);
+        double* dva = rc ? nullptr : alloc_fields(x, 4, rc);
         if (!rc) {
             x->d = prim; x->u = prim + x->fs; x->v = prim + 2 * x->fs; x->w = prim + 3 * x->fs;
             x->T = prim + 4 * x->fs; x->P = prim + 5 * x->fs;
-            x->du = dua; x->amu = am; x->ev = ev;
+            x->du = dua; x->amu = am; x->ev = ev; x->dv = dva;
         }
     } else {
         float* prim = rc ? nullptr : alloc_fields_f32(x, 6, rc);
@@ -1097,6 +1107,15 @@ int ocfd_get_du(ocfd_ctx* x, double* du5) {
     for (int m = 0; m < 5; m++)
         TRY(x->mixed ? download_any(x, x->sdu + m * x->fs, du5 + m * n)
                      : download_any(x, x->du + m * x->fs, du5 + m * n));
+    // the viscous part of components 1..4 when it is kept apart, added as
+    // the RK stage adds it
+    if (!x->mixed) {
+        std::vector<double> dv(n);
+        for (int m = 1; m < 5; m++) {
+            TRY(download_any(x, x->dv + (m - 1) * x->fs, dv.data()));
+            for (size_t p = 0; p < n; p++) du5[m * n + p] += dv[p];
+        }
+    }
     return 0;
 }
 
@@ -1470,7 +1489,7 @@ int ocfd_get_q(ocfd_ctx* x, double* q) {
         for (int m = 0; m < 6; m++) va.bnd[m] = C.dif_bound[m];
         va.u = x->su; va.v = x->sv; va.w = x->sw; va.T = x->sT; va.amu = x->samu;
         for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-        va.ev = x->sev; va.du = x->sdu; va.fs = x->fs;
+        va.ev = x->sev; va.du = x->sdu; va.dv = x->sdv; va.fs = x->fs;
         TRY(timed(x, "qcrit", [&] { launch_qcrit(va, out, x->st); }));
     } else {
         ViscArgsT<double> va;
@@ -1478,7 +1497,7 @@ int ocfd_get_q(ocfd_ctx* x, double* q) {
         for (int m = 0; m < 6; m++) va.bnd[m] = C.dif_bound[m];
         va.u = x->u; va.v = x->v; va.w = x->w; va.T = x->T; va.amu = x->amu;
         for (int m = 0; m < M_COUNT; m++) va.M[m] = x->M[m];
-        va.ev = x->ev; va.du = x->du; va.fs = x->fs;
+        va.ev = x->ev; va.du = x->du; va.dv = x->dv; va.fs = x->fs;
         TRY(timed(x, "qcrit", [&] { launch_qcrit(va, out, x->st); }));
     }
     return download_interior(x, out, q);

@@ -43,12 +43,15 @@ struct ViscArgsT {
     const double *M[M_COUNT];
     S *ev;                                   // 12 fields: Ev1..4 for x, y, z
     S *du;                                   // 5 fields
+    S *dv;                                   // 4 fields: the viscous part of du[1..4], or null (added to du)
     long long fs;
 };
 // Gradients of u,v,w,T (kept in registers) and the 12 viscous fluxes.
 template <typename S> void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st);
 // The divergence in launches of visc_div_components<S>() flux components
-// each, starting at component c0 (0..3).
+// each, starting at component c0 (0..3): written to dv when it is set (the
+// stage adds du + dv, so the divergence may run beside the inviscid sweeps),
+// else added to du after the sweeps.
 template <typename S> int visc_div_components();
 template <typename S> void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st);
 // Q criterion of u, v, w at every interior point into q (LAP layout, FP64).
@@ -69,6 +72,7 @@ struct StageArgsT {
     double *f;
     const double *fn;
     const S *du;
+    const S *dv;                             // 4 fields, the viscous part of du[1..4], or null
     double *flap;                            // 5 fields or null (written in stage 3 when non-null)
     S *d, *u, *v, *w, *T, *P, *amu;
     long long fs;
