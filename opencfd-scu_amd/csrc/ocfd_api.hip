@@ -510,7 +510,10 @@ StageArgsT<S> stage_args(ocfd_ctx* x, int krk, bool write_flap) {
     sa.g = x->g; sa.c = x->c; sa.krk = krk; sa.dt = x->cfg.dt;
     sa.f = x->f; sa.fn = x->fn; sa.du = V.du; sa.dv = V.dv;
     sa.flap = write_flap ? x->flap : nullptr;
-    sa.d = V.d; sa.u = V.u; sa.v = V.v; sa.w = V.w; sa.T = V.T; sa.P = V.P; sa.amu = V.amu;
+    sa.d = V.d; sa.u = V.u; sa.v = V.v; sa.w = V.w; sa.T = V.T; sa.amu = V.amu;
+    // the pressure is read only after a step (filter, get_pressure, analysis):
+    // the first two stages skip it
+    sa.P = krk == 3 ? V.P : nullptr;
     sa.fs = x->fs;
     return sa;
 }
@@ -2014,11 +2017,14 @@ void OCFD_time_advance(int KRK) {
 
 void get_duvwT(void) {
     ocfd_ctx* x = g_ctx;
+    // get_duvwT also computes the pressure
     if (x->mixed) {
-        const StageArgsT<float> sa = stage_args<float>(x, 1, false);
+        StageArgsT<float> sa = stage_args<float>(x, 1, false);
+        sa.P = x->sP;
         SHIM(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
     } else {
-        const StageArgsT<double> sa = stage_args<double>(x, 1, false);
+        StageArgsT<double> sa = stage_args<double>(x, 1, false);
+        sa.P = x->P;
         SHIM(timed(x, "c2p", [&] { launch_c2p_all(sa, x->st); }));
     }
 }

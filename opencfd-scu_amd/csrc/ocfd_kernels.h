@@ -74,7 +74,8 @@ struct StageArgsT {
     const S *du;
     const S *dv;                             // 4 fields, the viscous part of du[1..4], or null
     double *flap;                            // 5 fields or null (written in stage 3 when non-null)
-    S *d, *u, *v, *w, *T, *P, *amu;
+    S *d, *u, *v, *w, *T, *amu;
+    S *P;                                    // pressure, or null (the stage then skips it)
     long long fs;
 };
 template <typename S> void launch_stage(const StageArgsT<S>& a, hipStream_t st);
@@ -109,7 +110,7 @@ __device__ __forceinline__ void stage_point(const StageArgsT<S>& A, long long q,
     const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
     const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
     A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
-    A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
+    if (A.P) A.P[q] = S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama));
     A.amu[q] = S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2));
 }
 

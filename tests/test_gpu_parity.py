@@ -159,6 +159,9 @@ def test_ramp_flags_rhs_and_steps():
     assert (e <= TOL).all(), e
     for a, b in zip(g2.get_flags(), o2.get_flags()):
         assert np.array_equal(a, b)
+    # the pressure of the last stage (the filter's input), faces included
+    pg, po = g2.get_pressure(), o2.get_pressure()
+    assert np.abs(pg - po).max() / np.abs(po).max() <= TOL
 
 
 def test_ramp_wall_and_inlet_boundary_values():
