@@ -1948,6 +1948,30 @@ int orc_get_state(orc *o, double *f5) { memcpy(f5, o->f, sizeof(double) * NV * o
 int orc_get_du(orc *o, double *du5) { memcpy(du5, o->du, sizeof(double) * NV * o->nint); return 0; }
 int orc_get_pressure(orc *o, double *p) { get_interior(o, o->P, p); return 0; }
 int orc_get_pressure_lap(orc *o, double *p) { memcpy(p, o->P, sizeof(double) * o->nlap); return 0; }
+/* primitive k (0..4 = d,u,v,w,T) over the whole LAP box, ghosts included */
+int orc_get_prim_lap(orc *o, int k, double *a) {
+    const double *src[5] = {o->d, o->u, o->v, o->w, o->T};
+    if (k < 0 || k > 4) return -1;
+    memcpy(a, src[k], sizeof(double) * o->nlap);
+    return 0;
+}
+/* The filter alone on the current state, for the pins: f_lap <- f (as the
+ * third RK stage leaves it), p and the primitives from f (cons_to_pri), then
+ * filtering with every filter due. */
+int orc_filter_now(orc *o) {
+    long N = o->nint;
+    for (int m = 0; m < NV; m++)
+        for (int z = 0; z < o->nz; z++)
+            for (int y = 0; y < o->ny; y++)
+                for (int x = 0; x < o->nx; x++)
+                    o->f_lap[m * o->nlap + IL(o, x + LAP, y + LAP, z + LAP)] = o->f[I0(o, x, y, z) + m * N];
+    cons_to_pri(o);
+    int save = o->istep;
+    o->istep = 0;
+    orc_filtering(o);
+    o->istep = save;
+    return 0;
+}
 int orc_get_metric_lap(orc *o, int k, double *a) { memcpy(a, o->A[k], sizeof(double) * o->nlap); return 0; }
 int orc_get_amu(orc *o, double *amu) { memcpy(amu, o->Amu, sizeof(double) * o->nint); return 0; }
 int orc_get_metrics(orc *o, double *a13) {

@@ -45,6 +45,8 @@ int orc_get_du(orc *o, double *du5);
 int orc_get_pressure(orc *o, double *p);
 /* The pressure on the whole LAP box ((nz+8)(ny+8)(nx+8), x fastest). */
 int orc_get_pressure_lap(orc *o, double *p);
+int orc_get_prim_lap(orc *o, int k, double *a);
+int orc_filter_now(orc *o);
 /* Metric array k (0..12: Axx Ayy Azz Akx ... Ajac) over the LAP box. */
 int orc_get_metric_lap(orc *o, int k, double *a);
 int orc_get_amu(orc *o, double *amu);

@@ -57,6 +57,8 @@ def lib() -> C.CDLL:
             "orc_get_du": (C.c_int, [vp, dp]),
             "orc_get_pressure": (C.c_int, [vp, dp]),
             "orc_get_pressure_lap": (C.c_int, [vp, dp]),
+            "orc_get_prim_lap": (C.c_int, [vp, C.c_int, dp]),
+            "orc_filter_now": (C.c_int, [vp]),
             "orc_get_metric_lap": (C.c_int, [vp, C.c_int, dp]),
             "orc_get_amu": (C.c_int, [vp, dp]),
             "orc_get_metrics": (C.c_int, [vp, dp]),
@@ -167,6 +169,16 @@ class Oracle:
         out = out.reshape(self.nz + 8, self.ny + 8, self.nx + 8)
         c = 4 - ghosts
         return out[c:out.shape[0] - c, c:out.shape[1] - c, c:out.shape[2] - c]
+
+    def get_prim_lap(self, k):
+        """Primitive k (0..4 = d,u,v,w,T) with all 4 ghost layers, [z][y][x]."""
+        out = np.empty((self.nx + 8) * (self.ny + 8) * (self.nz + 8))
+        self.L.orc_get_prim_lap(self.h, int(k), out)
+        return out.reshape(self.nz + 8, self.ny + 8, self.nx + 8)
+
+    def filter_now(self):
+        """The filter alone on the current state (every filter due)."""
+        self.L.orc_filter_now(self.h)
 
     def get_metric_lap(self, k):
         """Metric array k (0..12) over the whole LAP box (nz+8, ny+8, nx+8)."""

@@ -14,7 +14,10 @@ dropped term) fails one of them:
 * telescoping conservation of the periodic RHS and exact freestream
   preservation on a uniform grid,
 * metric identities on an affine mesh, Sutherland mu(1) = 1/Re,
-* Jameson sensor zeros on linear pressure.
+* Jameson sensor zeros on linear pressure,
+* the IBC 108 / IBC 124 wall extrapolations on polynomial profiles, the
+  inlet / top / upper tables and the mirrored ghost layers,
+* the filter stencil with the shock sensor forced to 1.
 """
 import math
 
@@ -446,3 +449,183 @@ def test_filter_with_zero_sigma_leaves_interior_unchanged():
     assert np.allclose(fb, fc, rtol=1e-13, atol=1e-13)
     a.step(1)
     assert not np.array_equal(a.get_state(), fc)  # the real filter does act
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions.  The wall values are one-sided extrapolations in the
+# wall-normal index with zero slope at the wall: IBC 108 uses
+# p_w = (18 p_1 - 9 p_2 + 2 p_3)/11 (exact for a + c j^2 + e j^3; a + b j
+# gives a + 6b/11), IBC 124 uses (4 p_1 - p_2)/3 (exact for a + c k^2; a + b k
+# gives a + 2b/3).  Inlet, top and upper faces take the tables given; ghost
+# layers mirror the interior with the velocity odd at a no-slip wall and the
+# normal velocity odd at a symmetry plane.
+# ---------------------------------------------------------------------------
+def _wall_rows(shape_xz, rng):
+    a = 1.0 + 0.2 * rng.random(shape_xz)
+    return a, 0.05 * rng.random(shape_xz), 0.01 * rng.random(shape_xz)
+
+
+@pytest.mark.parametrize("form", ["cubic", "linear"])
+def test_ibc108_isothermal_wall_pressure_extrapolation(form):
+    c = cases.ramp(24, 16, 6, nsteps=1, filt=False)
+    c.cfg.bc_rpara[1] = 0.0  # no wall blowing
+    tw = c.cfg.bc_rpara[0]
+    d, u, v, w, T = [p.copy() for p in c.prims]
+    a, cq, e = _wall_rows((6, 24), np.random.default_rng(3))
+    for j in (1, 2, 3):
+        T[:, j, :] = 2.0 + 0.1 * j
+        p = a + cq * j ** 2 + e * j ** 3 if form == "cubic" else a + cq * j
+        d[:, j, :] = p / T[:, j, :]
+    c.prims = [d, u, v, w, T]
+    o = c.load(O.Oracle(c.cfg))
+    (d2, u2, v2, w2, T2), _, _ = o.get_flow()
+    # i = 0 takes the inlet profile before the wall is set: the wall there
+    # extrapolates inlet values
+    pw = d2[:, 0, 1:] * T2[:, 0, 1:]
+    want = a[:, 1:] if form == "cubic" else a[:, 1:] + 6.0 * cq[:, 1:] / 11.0
+    assert np.array_equal(T2[:, 0, :], np.full((6, 24), tw))
+    assert np.abs(pw - want).max() <= 1e-14 * np.abs(want).max()
+    assert not u2[:, 0, :].any() and not v2[:, 0, :].any() and not w2[:, 0, :].any()
+
+
+def test_ibc108_adiabatic_wall_extrapolation():
+    c = cases.ramp(24, 16, 6, nsteps=1, filt=False)
+    c.cfg.bc_rpara[1] = 0.0
+    c.cfg.bc_rpara[0] = -1.0  # TW <= 0: adiabatic
+    d, u, v, w, T = [p.copy() for p in c.prims]
+    rng = np.random.default_rng(4)
+    a, cq, e = _wall_rows((6, 24), rng)
+    t0, tq, te = 2.0 + rng.random((6, 24)), 0.02 * rng.random((6, 24)), 0.01 * rng.random((6, 24))
+    for j in (1, 2, 3):
+        T[:, j, :] = t0 + tq * j ** 2 + te * j ** 3
+        d[:, j, :] = (a + cq * j ** 2 + e * j ** 3) / T[:, j, :]
+    c.prims = [d, u, v, w, T]
+    o = c.load(O.Oracle(c.cfg))
+    (d2, u2, v2, w2, T2), _, _ = o.get_flow()
+    assert np.abs(T2[:, 0, 1:] - t0[:, 1:]).max() <= 1e-14 * t0.max()
+    assert np.abs(d2[:, 0, 1:] * T2[:, 0, 1:] - a[:, 1:]).max() <= 1e-14 * a.max()
+
+
+def test_ibc108_inlet_top_and_wall_ghosts():
+    c = cases.ramp(24, 16, 6, nsteps=1, filt=False)
+    o = c.load(O.Oracle(c.cfg))
+    (d2, u2, v2, w2, T2), _, _ = o.get_flow()
+    prof = c.inlet  # rho, u, v, T over j
+    # inlet column (between the wall and the top row)
+    for got, want in ((d2, prof[0]), (u2, prof[1]), (v2, prof[2]), (T2, prof[3])):
+        assert np.array_equal(got[:, 1:-1, 0], np.broadcast_to(want[1:-1], (6, 14)))
+    assert not w2[:, 1:-1, 0].any()
+    # top row from x_begin (0 here: no top x below XUB) takes the profile's top
+    for got, want in ((d2, prof[0]), (u2, prof[1]), (v2, prof[2]), (T2, prof[3])):
+        assert np.array_equal(got[:, -1, :], np.full((6, 24), want[-1]))
+    # ghosts below the wall: rho, T even, velocity odd about j = 0
+    L = 4
+    for k, sgn in ((0, 1.0), (1, -1.0), (2, -1.0), (3, -1.0), (4, 1.0)):
+        a = o.get_prim_lap(k)
+        for m in range(1, 5):
+            assert np.array_equal(a[L:-L, L - m, L:-L], sgn * a[L:-L, L + m, L:-L])
+
+
+@pytest.mark.parametrize("form", ["quadratic", "linear"])
+def test_ibc124_isothermal_wall_pressure_extrapolation(form):
+    c = cases.liftbody(20, 12, 10, nsteps=1, tw=1.5)
+    c.cfg.bc_rpara[2] = 0.0  # EPSL_WALL: no wall disturbance
+    c.cfg.bc_rpara[3] = 0.0  # EPSL_UPPER
+    d, u, v, w, T = [p.copy() for p in c.prims]
+    a, cq, _ = _wall_rows((12, 20), np.random.default_rng(5))
+    for k in (1, 2):
+        T[k] = 1.2 + 0.1 * k
+        p = a + cq * k ** 2 if form == "quadratic" else a + cq * k
+        d[k] = p / T[k]
+    c.prims = [d, u, v, w, T]
+    o = c.load(O.Oracle(c.cfg))
+    (d2, u2, v2, w2, T2), _, _ = o.get_flow()
+    want = a if form == "quadratic" else a + 2.0 * cq / 3.0
+    # i = 0 is the inlet, whose values are set before the wall
+    assert np.array_equal(T2[0, :, 1:], np.full((12, 19), 1.5))
+    assert np.abs(d2[0, :, 1:] * 1.5 - want[:, 1:]).max() <= 1e-14 * want.max()
+    assert not u2[0, :, 1:].any() and not v2[0, :, 1:].any() and not w2[0, :, 1:].any()
+
+
+def test_ibc124_adiabatic_wall_upper_and_symmetry_ghosts():
+    c = cases.liftbody(20, 12, 10, nsteps=1, tw=-1.0)
+    c.cfg.bc_rpara[2] = 0.0
+    c.cfg.bc_rpara[3] = 0.0
+    d, u, v, w, T = [p.copy() for p in c.prims]
+    rng = np.random.default_rng(6)
+    t0, tq = 2.0 + rng.random((12, 20)), 0.05 * rng.random((12, 20))
+    d0, dq = 0.5 + rng.random((12, 20)), 0.05 * rng.random((12, 20))
+    for k in (1, 2):
+        T[k] = t0 + tq * k ** 2
+        d[k] = d0 + dq * k ** 2
+    c.prims = [d, u, v, w, T]
+    o = c.load(O.Oracle(c.cfg))
+    (d2, u2, v2, w2, T2), _, _ = o.get_flow()
+    assert np.abs(T2[0, :, 1:] - t0[:, 1:]).max() <= 1e-14 * t0.max()
+    assert np.abs(d2[0, :, 1:] - d0[:, 1:]).max() <= 1e-14 * d0.max()
+    # upper face without a table or disturbance: the free stream at AOA
+    aoa = math.radians(c.cfg.bc_rpara[0])
+    for got, want in ((d2, 1.0), (u2, math.cos(aoa)), (v2, 0.0), (w2, math.sin(aoa)), (T2, 1.0)):
+        assert np.array_equal(got[-1, :, 1:], np.full((12, 19), want))
+    # symmetry planes j = 0 and j = ny-1 (the plane y = 0): v odd, the rest even
+    L = 4
+    for k, sgn in ((0, 1.0), (1, 1.0), (2, -1.0), (3, 1.0), (4, 1.0)):
+        a = o.get_prim_lap(k)
+        for m in range(1, 5):
+            assert np.array_equal(a[L:-L, L - m, L:-L], sgn * a[L:-L, L + m, L:-L])
+            assert np.array_equal(a[L:-L, L + 11 + m, L:-L], sgn * a[L:-L, L + 11 - m, L:-L])
+
+
+def test_ibc124_inlet_and_upper_tables():
+    c = cases.liftbody(20, 12, 10, nsteps=1, withleading=1, upper=1)
+    c.cfg.bc_rpara[3] = 0.0  # no upper disturbance: the upper face is the table
+    o = c.load(O.Oracle(c.cfg))
+    prims, _, _ = o.get_flow()
+    inl, up = c.extra["inlet5"], c.extra["upper5"]
+    for m in range(5):
+        # inlet i = 0 between the wall (k = 0) and the upper face
+        assert np.array_equal(prims[m][1:-1, :, 0], inl[m][1:-1])
+        assert np.array_equal(prims[m][-1, :, 1:], up[m][:, 1:])
+
+
+# ---------------------------------------------------------------------------
+# Filter: with RTH = 0 the shock sensor is 1 everywhere and the filter is the
+# fixed stencil f_i - s0 [c1 (f_{i+1} - 2 f_i + f_{i-1})
+# + c2 (f_{i+2} - f_{i+1} - f_{i-1} + f_{i-2})].  On f = i^2 the bracket is
+# 2 c1 + 6 c2; on f = (i - i0)^4 at i0 it is 2 c1 + 30 c2: the two pin c1 and
+# c2 (c1 = -0.2103830, c2 = 0.0396170); a linear f is left unchanged.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("form", ["quadratic", "quartic", "linear"])
+def test_filter_stencil_with_unit_sensor(form):
+    nx = 40
+    c = cases.ramp(nx, 24, 8, nsteps=1, filter_every=1)
+    c.cfg.filter_rpara[0][1] = 0.0  # RTH = 0: sigma = 1
+    c.cfg.filter_npara[0][2] = 0    # x only
+    c.cfg.filter_npara[0][3] = 0
+    d, u, v, w, T = [p.copy() for p in c.prims]
+    i = np.arange(nx, dtype=float)
+    i0 = 20
+    beta = {"quadratic": 1e-4, "quartic": 1e-6, "linear": 1e-3}[form]
+    prof = {"quadratic": (i - i0) ** 2, "quartic": (i - i0) ** 4, "linear": i - i0}[form]
+    d[:] = 1.0 + beta * prof
+    u[:] = 0.7
+    v[:] = 0.1
+    w[:] = 0.0
+    T[:] = 2.0
+    c.prims = [d, u, v, w, T]
+    o = c.load(O.Oracle(c.cfg))
+    f0 = o.get_state()
+    o.filter_now()
+    f1 = o.get_state()
+    s0 = c.cfg.filter_rpara[0][0]
+    c1, c2 = -0.2103830, 0.0396170
+    bracket = {"quadratic": 2 * c1 + 6 * c2, "quartic": 2 * c1 + 30 * c2, "linear": 0.0}[form]
+    # away from the wall, inlet and top rows; each conserved variable is a
+    # constant multiple of rho here
+    for m in range(5):
+        scale = f0[m][3, 8, i0] / d[3, 8, i0]
+        got = f1[m][2:-2, 6:-6, i0] - f0[m][2:-2, 6:-6, i0]
+        want = -s0 * bracket * beta * scale
+        assert np.abs(got - want).max() <= 1e-12 * max(abs(scale), 1e-300), (m, got.max(), want)
+    if form == "linear":
+        assert np.abs(f1[0][2:-2, 6:-6, 8:-8] - f0[0][2:-2, 6:-6, 8:-8]).max() <= 1e-15
