@@ -164,6 +164,22 @@ def test_ramp_flags_rhs_and_steps():
     assert np.abs(pg - po).max() / np.abs(po).max() <= TOL
 
 
+def test_ramp_long_run():
+    """60 steps (180 right-hand sides, 6 filter passes) of the ramp with its
+    shock, closures, IBC 108 and non-reflecting faces: the rounding
+    differences (FMA contraction) must not grow past the bar."""
+    case = cases.ramp(48, 32, 10, nsteps=60, filter_every=10)
+    g, o = pair(case)
+    for n in (20, 20, 20):
+        g.step(n)
+        o.step(n)
+        e = relerr(g.get_state(), o.get_state())
+        report(f"ramp after {g.get_flow()[1]} steps", e)
+        assert (e <= TOL).all(), e
+    for a, b in zip(g.get_flags(), o.get_flags()):
+        assert np.array_equal(a, b)
+
+
 def test_ramp_wall_and_inlet_boundary_values():
     case = cases.ramp(40, 24, 8, nsteps=3, filt=False)
     g, o = pair(case)
