@@ -257,13 +257,18 @@ __device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, 
     face_all_m<R, W, SCH>(ld, c, r, bl, br, hy, hm);
 }
 
-// x sweep: a block owns 4 rows of 56 output nodes; the f+/f- of the 64 nodes
-// of a row (4 ghost nodes each side) are staged in LDS.
-constexpr int XS = 56, XA = 4, XN = XS + 8, XF = XS + 1;
+// x sweep: a block owns XA rows of 56 output nodes (one wave per row); the
+// f+/f- of the 64 nodes of a row (4 ghost nodes each side) are staged in LDS.
+#ifndef OCFD_XA
+// rows per x-sweep block (one wave each): measured 146 us with 1 row, 156 us
+// with 2 and 158 us with 4 (FP64, 512x128x64 ramp, kernel alone)
+#define OCFD_XA 1
+#endif
+constexpr int XS = 56, XA = OCFD_XA, XN = XS + 8, XF = XS + 1;
 
 template <int SCH, bool FIRST, typename S>
-__global__ void __launch_bounds__(256) k_sweep(SweepArgsT<S> A) {
-    static_assert(XF <= 64 && XA == 4 && XN * XA == 256, "one row per wave; one node and one face per thread");
+__global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
+    static_assert(XF <= 64 && XN == 64, "one row per wave; one node and one face per thread");
     __shared__ S sh[10 * XN * XA];
     const Geo& g = A.g;
     const int tid = threadIdx.x;
@@ -857,8 +862,8 @@ void launch_dir(const SweepArgsT<S>& a, bool first, hipStream_t st) {
     const Geo& g = a.g;
     if constexpr (DIR == 0) {
         dim3 grid((g.nx + XS - 1) / XS, (g.ny + XA - 1) / XA, g.nz);
-        if (first) hipLaunchKernelGGL((k_sweep<SCH, true, S>), grid, dim3(256), 0, st, a);
-        else hipLaunchKernelGGL((k_sweep<SCH, false, S>), grid, dim3(256), 0, st, a);
+        if (first) hipLaunchKernelGGL((k_sweep<SCH, true, S>), grid, dim3(64 * XA), 0, st, a);
+        else hipLaunchKernelGGL((k_sweep<SCH, false, S>), grid, dim3(64 * XA), 0, st, a);
     } else {
         const int n_dir = DIR == 1 ? g.ny : g.nz, n_acr = DIR == 1 ? g.nz : g.ny;
         if constexpr (std::is_same<S, float>::value && OCFD_MARCH_F32) {
