@@ -73,9 +73,10 @@ __device__ __forceinline__ void p2c(const BCArgsT<S>& A, long long q) {
 // ---------------------------------------------------------------------------
 // IBC 108, bc_user_Compression_conner (src/OCFD_boundary_compression_conner.cu
 // 27-179: inlet, top, wall velocity, wall temperature/density with mirrored
-// ghosts), get_Amu on the faces it changes, and the face refresh, in three
-// launches instead of the reference's eight, with the reference's order kept
-// where points are shared: (A) inlet and top (they
+// ghosts), get_Amu on the faces it changes, and the face refresh.  By
+// default one launch does all of it (k_bc108_faces below); the three
+// launches here keep the reference's order where points are shared, and
+// are the path the merged kernel is checked against: (A) inlet and top (they
 // write the same values where they overlap), (B) per wall column the wall
 // velocity then the wall temperature/density and mirrored ghosts (which read
 // the inlet column at i = 0, so B follows A), (C) per face point the viscosity
@@ -163,6 +164,120 @@ __global__ void k_faces(BCArgsT<S> A) {
         A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
     }
     p2c(A, q);
+}
+
+// ---------------------------------------------------------------------------
+// IBC 108 and the face refresh in one launch.  A thread of a face point
+// derives the point's primitives after the boundary condition from the
+// state before it, by the rules and in the order of (A) and (B) above (inlet
+// columns i = -4..0, then the top row from x_begin, then the wall row j = 0
+// with rows 1..3 as (A) left them), writes them, and refreshes the point as
+// (C) does.  The ghost layers the condition writes (inlet i = -4..-1, wall
+// j = -4..-1) have threads of their own.  A point on two faces is done by
+// two threads with the same values; a thread reads from memory only points
+// the condition leaves unchanged.
+// ---------------------------------------------------------------------------
+template <typename S> struct Prim5 { S d, u, v, w, T; };
+
+// primitives of (i, j, k) after (A)
+template <typename S>
+__device__ __forceinline__ Prim5<S> bc108_after_a(const BCArgsT<S>& A, int i, int j, int k) {
+    const int ny = A.g.ny;
+    Prim5<S> p;
+    if (i <= 0 && i >= -4 && j >= 0 && j < ny) {
+        p.d = S(A.ub1[j]); p.u = S(A.ub1[j + ny]); p.v = S(A.ub1[j + 2 * ny]); p.w = S(0); p.T = S(A.ub1[j + 3 * ny]);
+    } else if (j == ny - 1 && i >= A.x_begin && i < A.g.nx) {
+        p.d = S(A.ub1[ny - 1]); p.u = S(A.ub1[ny + ny - 1]); p.v = S(A.ub1[2 * ny + ny - 1]); p.w = S(0);
+        p.T = S(A.ub1[3 * ny + ny - 1]);
+    } else {
+        const long long q = gidx(A.g, i, j, k);
+        p.d = A.d[q]; p.u = A.u[q]; p.v = A.v[q]; p.w = A.w[q]; p.T = A.T[q];
+    }
+    return p;
+}
+// primitives of (i, j, k) after (A) and (B)
+template <typename S>
+__device__ __forceinline__ Prim5<S> bc108_final(const BCArgsT<S>& A, int i, int j, int k) {
+    Prim5<S> p = bc108_after_a(A, i, j, k);
+    if (j != 0 || i < 0 || i >= A.g.nx) return p;
+    p.u = S(0);
+    p.v = S(A.epsl * A.fx[i] * A.gz[k] * A.ht);
+    p.w = S(0);
+    if (A.wall_not_normal != 0) return p;
+    const Prim5<S> p1 = bc108_after_a(A, i, 1, k), p2 = bc108_after_a(A, i, 2, k), p3 = bc108_after_a(A, i, 3, k);
+    const double D1 = p1.d, D2 = p2.d, D3 = p3.d, T1 = p1.T, T2 = p2.T, T3 = p3.T;
+    if (A.tw > 0) {
+        const double pw = (18. * D1 * T1 - 9. * D2 * T2 + 2. * D3 * T3) / 11.;
+        p.T = S(A.tw);
+        p.d = S(pw / double(p.T));
+    } else {
+        p.T = S((18. * T1 - 9. * T2 + 2. * T3) / 11.);
+        const double pw = (18. * D1 * T1 - 9. * D2 * T2 + 2. * D3 * T3) / 11.;
+        p.d = S(pw / double(p.T));
+    }
+    return p;
+}
+template <typename S>
+__device__ __forceinline__ void put_prim(const BCArgsT<S>& A, long long q, const Prim5<S>& p) {
+    A.d[q] = p.d; A.u[q] = p.u; A.v[q] = p.v; A.w[q] = p.w; A.T[q] = p.T;
+}
+template <typename S>
+__global__ void k_bc108_faces(BCArgsT<S> A) {
+    const Geo& g = A.g;
+    const long long nyz = (long long)g.ny * g.nz, nxz = (long long)g.nx * g.nz, nxy = (long long)g.nx * g.ny;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 4 * nyz) {  // inlet ghosts i = -4..-1
+        const int i = -4 + (int)(t / nyz);
+        const long long r = t % nyz;
+        const int j = (int)(r % g.ny), k = (int)(r / g.ny);
+        put_prim(A, gidx(g, i, j, k), bc108_after_a(A, i, j, k));
+        return;
+    }
+    t -= 4 * nyz;
+    const bool ghosts = A.tw > 0 && A.wall_not_normal == 0;
+    if (t < 4 * nxz) {  // wall ghosts j = -4..-1 mirror rows 4..1 (rho, T even; velocity odd)
+        if (!ghosts) return;
+        const int l = (int)(t / nxz);
+        const long long r = t % nxz;
+        const int i = (int)(r % g.nx), k = (int)(r / g.nx);
+        const Prim5<S> m = bc108_after_a(A, i, 4 - l, k);
+        Prim5<S> p;
+        p.d = m.d; p.u = -m.u; p.v = -m.v; p.w = -m.w; p.T = m.T;
+        put_prim(A, gidx(g, i, l - 4, k), p);
+        return;
+    }
+    t -= 4 * nxz;
+    int i, j, k;
+    bool mu = false;
+    if (t < 2 * nyz) {
+        const int side = (int)(t / nyz);
+        const long long r = t % nyz;
+        i = side ? g.nx - 1 : 0; j = (int)(r % g.ny); k = (int)(r / g.ny);
+        mu = (A.mu_faces >> side) & 1;
+    } else if ((t -= 2 * nyz) < 2 * nxz) {
+        const int side = (int)(t / nxz);
+        const long long r = t % nxz;
+        i = (int)(r % g.nx); j = side ? g.ny - 1 : 0; k = (int)(r / g.nx);
+        mu = (A.mu_faces >> (2 + side)) & 1;
+    } else if ((t -= 2 * nxz) < 2 * nxy) {
+        const int side = (int)(t / nxy);
+        const long long r = t % nxy;
+        i = (int)(r % g.nx); j = (int)(r / g.nx); k = side ? g.nz - 1 : 0;
+        mu = (A.mu_faces >> (4 + side)) & 1;
+    } else {
+        return;
+    }
+    const long long q = gidx(g, i, j, k);
+    const Prim5<S> p = bc108_final(A, i, j, k);
+    put_prim(A, q, p);
+    const double T = p.T;
+    if (mu && A.amu) A.amu[q] = S(A.c.amu_c0 * sqrt(T * T * T) / (A.c.tsb + T));
+    const double d = p.d, u = p.u, v = p.v, w = p.w;
+    A.f[q] = d;
+    A.f[A.fs + q] = u * d;
+    A.f[2 * A.fs + q] = v * d;
+    A.f[3 * A.fs + q] = w * d;
+    A.f[4 * A.fs + q] = d * ((u * u + v * v + w * w) * 0.5 + A.c.cv * T);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,6 +649,14 @@ void launch_bc108(const BCArgsT<S>& a, hipStream_t st) {
     hipLaunchKernelGGL(k_bc_wall<S>, dim3((g.nx + 63) / 64, g.nz), dim3(64), 0, st, a);
 }
 
+template <typename S>
+void launch_bc108_faces(const BCArgsT<S>& a, hipStream_t st) {
+    const Geo& g = a.g;
+    const long long tot = 4LL * g.ny * g.nz + 4LL * g.nx * g.nz + 2LL * g.ny * g.nz + 2LL * g.nx * g.nz +
+                          2LL * g.nx * g.ny;
+    hipLaunchKernelGGL(k_bc108_faces<S>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a);
+}
+
 template <typename T>
 void launch_mirror_y(const Geo& g, T* f, double sign, hipStream_t st) {
     hipLaunchKernelGGL(k_mirror_y<T>, dim3((g.nx + 63) / 64, 8, g.nz), dim3(64), 0, st, g, f, sign);
@@ -627,6 +750,7 @@ void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, 
     template void launch_fill<S>(const Geo&, S*, int, long long, int, hipStream_t);                          \
     template void launch_stage<S>(const StageArgsT<S>&, hipStream_t);                                        \
     template void launch_bc108<S>(const BCArgsT<S>&, hipStream_t);                                           \
+    template void launch_bc108_faces<S>(const BCArgsT<S>&, hipStream_t);                                     \
     template void launch_bc124<S>(const BCArgsT<S>&, int, hipStream_t);                                      \
     template void launch_mirror_y<S>(const Geo&, S*, double, hipStream_t);                                   \
     template void launch_face_refresh<S>(const BCArgsT<S>&, hipStream_t);                                    \

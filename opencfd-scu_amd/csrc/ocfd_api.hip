@@ -54,6 +54,7 @@ struct ocfd_ctx {
     hipStream_t st2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool serial = false;  // OCFD_SERIAL=1: one stream (per-kernel timing with an external profiler)
+    bool bc108_split = false;  // OCFD_BC108_SPLIT=1: IBC 108 and the face refresh in three launches
     // pinned staging buffer of the device-to-host downloads (grown on demand)
     void* pin = nullptr;
     size_t pin_bytes = 0;
@@ -382,6 +383,12 @@ int apply_bc(ocfd_ctx* x) {
             ht = 1.;
         }
         a.ht = ht;
+        // IBC 108 and the face refresh in one launch (OCFD_BC108_SPLIT=1 at
+        // ocfd_create: the three separate launches, the same values)
+        if (!x->bc108_split) {
+            TRY(timed(x, "bc108", [&] { launch_bc108_faces(a, x->st); }));
+            return 0;
+        }
         TRY(timed(x, "bc108", [&] { launch_bc108(a, x->st); }));
     } else if (x->cfg.ibc == 124) {
         // bc_user_Liftbody3d (src/OCFD_boundary_Liftbody3D.cu:221-229): the
@@ -868,6 +875,7 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
         return fail(OCFD_ERR_HIP, "hipStreamCreate / hipEventCreate failed");
     }
     if (const char* e = std::getenv("OCFD_SERIAL")) x->serial = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OCFD_BC108_SPLIT")) x->bc108_split = std::atoi(e) != 0;
     Geo& g = x->g;
     g.nx = C.nx; g.ny = C.ny; g.nz = C.nz;
     // row pitch: 128-byte rows in FP64; 32 elements (128 bytes) in FP32 too

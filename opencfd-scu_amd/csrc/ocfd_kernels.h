@@ -136,6 +136,9 @@ struct BCArgsT {
     double sin_aoa, cos_aoa;
 };
 template <typename S> void launch_bc108(const BCArgsT<S>& a, hipStream_t st);
+// IBC 108 followed by the face refresh, in one launch (the same values as
+// launch_bc108 then launch_face_refresh).
+template <typename S> void launch_bc108_faces(const BCArgsT<S>& a, hipStream_t st);
 // IBC 124 (bc_user_Liftbody3d, src/OCFD_boundary_Liftbody3D.cu:185-294):
 // inlet, upper face, wall, then the symmetry planes when sym != 0.
 template <typename S> void launch_bc124(const BCArgsT<S>& a, int sym, hipStream_t st);

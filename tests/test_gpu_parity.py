@@ -69,10 +69,9 @@ def test_rhs_closures_low_high_both(scheme, bounds, nonref):
 def test_hybrid_flag_mixes(intv, closed):
     """hybrid_auto with HY_DP_INTV chosen for about 4 %, 31 %, 50 % and all of
     the faces on WENO7 or WENO5 (64x24x12 wavy box, periodic or closed on
-    every face).  The sweeps reconstruct the WENO faces of a tile from a
-    per-block queue and, when it is full, in place: every mix and the
-    closures must give the oracle's flags and du; the FP32 bulk path is
-    held to 1e-4."""
+    every face): waves whose faces mix UP7, WENO7 and WENO5 run each branch
+    for their own lanes; every mix and the closures must give the oracle's
+    flags and du; the FP32 bulk path is held to 1e-4."""
     case = cases.wavy_box(64, 24, 12, scheme="SCHEME_HYBRIDAUTO", amp=0.08,
                           periodic=(0, 0, 0) if closed else (1, 1, 1))
     case.cfg.hy_dp_intv[0], case.cfg.hy_dp_intv[1] = intv
@@ -194,6 +193,34 @@ def test_ramp_wall_and_inlet_boundary_values():
     assert np.array_equal(pg[0][:, 1:, 0], np.broadcast_to(case.inlet[0][1:], pg[0][:, 1:, 0].shape))
     assert np.array_equal(pg[0][:, 1:, 0], po[0][:, 1:, 0])
     assert relerr(np.stack(pg), np.stack(po)).max() <= TOL
+
+
+@pytest.mark.parametrize("variant", ["adiabatic", "not_normal", "x_begin", "split"])
+def test_ibc108_variants(variant, monkeypatch):
+    """IBC 108 paths beside the isothermal ramp: adiabatic wall (TW <= 0),
+    wall velocity only (BC_NPARA(4) != 0), a top row that starts at x_begin
+    > 0 (XUB), and the three separate boundary launches (OCFD_BC108_SPLIT=1)
+    instead of the merged one: 3 steps against the oracle, primitives and
+    state."""
+    case = cases.ramp(40, 24, 8, nsteps=3, filt=False)
+    if variant == "adiabatic":
+        case.cfg.bc_rpara[0] = -1.0
+    elif variant == "not_normal":
+        case.cfg.bc_npara[3] = 1
+    elif variant == "x_begin":
+        case.cfg.bc_rpara[6] = 0.3
+    elif variant == "split":
+        monkeypatch.setenv("OCFD_BC108_SPLIT", "1")
+    g, o = pair(case)
+    g.step(3)
+    o.step(3)
+    (pg, _, _), (po, _, _) = g.get_flow(), o.get_flow()
+    e = relerr(np.stack(pg), np.stack(po))
+    report(f"ibc108 {variant} prims", e)
+    assert (e <= TOL).all(), e
+    e = relerr(g.get_state(), o.get_state())
+    report(f"ibc108 {variant} state", e)
+    assert (e <= TOL).all(), e
 
 
 def test_full_size_tgv_mass_conservation():
