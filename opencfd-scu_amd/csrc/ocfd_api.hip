@@ -55,6 +55,7 @@ struct ocfd_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool serial = false;  // OCFD_SERIAL=1: one stream (per-kernel timing with an external profiler)
     bool bc108_split = false;  // OCFD_BC108_SPLIT=1: IBC 108 and the face refresh in three launches
+    bool graphs = true;        // OCFD_GRAPH=0: no HIP graph replay of steps
     // pinned staging buffer of the device-to-host downloads (grown on demand)
     void* pin = nullptr;
     size_t pin_bytes = 0;
@@ -876,6 +877,7 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     }
     if (const char* e = std::getenv("OCFD_SERIAL")) x->serial = std::atoi(e) != 0;
     if (const char* e = std::getenv("OCFD_BC108_SPLIT")) x->bc108_split = std::atoi(e) != 0;
+    if (const char* e = std::getenv("OCFD_GRAPH")) x->graphs = std::atoi(e) != 0;
     Geo& g = x->g;
     g.nx = C.nx; g.ny = C.ny; g.nz = C.nz;
     // row pitch: 128-byte rows in FP64; 32 elements (128 bytes) in FP32 too
@@ -1104,10 +1106,59 @@ int ocfd_rhs(ocfd_ctx* x, int krk) {
     return 0;
 }
 
+// Whether a step's launches have the same arguments every step, so that a
+// HIP graph of two steps (after which f and fn are back in place) can be
+// replayed: no filter (due every N steps), no time-dependent wall
+// disturbance (its factor ht follows tt), no kernel timing.
+bool steps_replayable(const ocfd_ctx* x) {
+    const ocfd_config& C = x->cfg;
+    if (!x->graphs || x->prof || x->serial || C.nfilter > 0) return false;
+    if ((C.ibc == 108 || C.ibc == 124) && C.bc_rpara[4] > 0.) return false;
+    return true;
+}
+
+// Steps [0, n) with n >= 4 as replays of one captured graph of two steps:
+// the capture runs the host side of two steps (swaps, counters) and records
+// their launches, the first replay executes them, each further replay
+// advances the counters as two steps do.  Returns the steps done (even).
+int step_graph(ocfd_ctx* x, int n, int& done) {
+    done = 0;
+    hipGraph_t gr = nullptr;
+    hipGraphExec_t ge = nullptr;
+    HIPCK(hipStreamBeginCapture(x->st, hipStreamCaptureModeRelaxed));
+    int rc = one_step(x);
+    if (!rc) rc = one_step(x);
+    const hipError_t ec = hipStreamEndCapture(x->st, &gr);
+    if (rc || ec != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return rc ? rc : fail(OCFD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    }
+    if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(gr);
+        return fail(OCFD_ERR_HIP, "hipGraphInstantiate failed");
+    }
+    hipError_t e = hipSuccess;
+    for (; done + 2 <= n && e == hipSuccess; done += 2) {
+        if (done > 0) {  // the host side of two more steps (the swaps cancel)
+            x->istep += 2;
+            x->tt += x->cfg.dt;
+            x->tt += x->cfg.dt;
+        }
+        e = hipGraphLaunch(ge, x->st);
+    }
+    (void)hipStreamSynchronize(x->st);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return fail(OCFD_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return 0;
+}
+
 int ocfd_step(ocfd_ctx* x, int nsteps) {
     if (!x || !x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
     HIPCK(hipEventRecord(x->step_a, x->st));
-    for (int s = 0; s < nsteps; s++) TRY(one_step(x));
+    int s = 0;
+    if (nsteps >= 4 && steps_replayable(x)) TRY(step_graph(x, nsteps, s));
+    for (; s < nsteps; s++) TRY(one_step(x));
     HIPCK(hipEventRecord(x->step_b, x->st));
     HIPCK(hipEventSynchronize(x->step_b));
     float ms = 0.f;
