@@ -112,6 +112,75 @@ __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
     return t;
 }
 
+// Gradients of u, v, w, T at (t.i, t.j, k) from the plane tile pl (the
+// current plane with its x and y strips, row pitch PW) and the thread's z
+// queue q, and at the same point the viscous fluxes Ev1..4 of the three
+// directions (vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker).
+template <int SV, typename S, int PW>
+__device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k, const S (*pl)[PW * (TY + 8)],
+                                         const S (*q)[9], int tx, int ty, S rhx, S rhy, S rhz) {
+    const Geo& g = A.g;
+    S gr[4][3];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        S s[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + 4) * PW + tx + m];
+        gr[f][0] = cd9<SV>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
+#pragma unroll
+        for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + m) * PW + tx + 4];
+        gr[f][1] = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
+        gr[f][2] = cd9<SV>(q[f], k, g.nz, A.bnd[4], A.bnd[5], rhz);
+    }
+    // vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker
+    const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
+    const S Akx = S(A.M[M_KX][qi]), Aky = S(A.M[M_KY][qi]), Akz = S(A.M[M_KZ][qi]);
+    const S Aix = S(A.M[M_IX][qi]), Aiy = S(A.M[M_IY][qi]), Aiz = S(A.M[M_IZ][qi]);
+    const S Asx = S(A.M[M_SX][qi]), Asy = S(A.M[M_SY][qi]), Asz = S(A.M[M_SZ][qi]);
+    const S Amu = A.amu[qi];
+    const S uk = gr[0][0], ui = gr[0][1], us = gr[0][2];
+    const S vk = gr[1][0], vi = gr[1][1], vs = gr[1][2];
+    const S wk = gr[2][0], wi = gr[2][1], ws = gr[2][2];
+    const S Tk = gr[3][0], Ti = gr[3][1], Ts = gr[3][2];
+    const S ux = uk * Akx + ui * Aix + us * Asx;
+    const S vx = vk * Akx + vi * Aix + vs * Asx;
+    const S wx = wk * Akx + wi * Aix + ws * Asx;
+    const S uy = uk * Aky + ui * Aiy + us * Asy;
+    const S vy = vk * Aky + vi * Aiy + vs * Asy;
+    const S wy = wk * Aky + wi * Aiy + ws * Asy;
+    const S uz = uk * Akz + ui * Aiz + us * Asz;
+    const S vz = vk * Akz + vi * Aiz + vs * Asz;
+    const S wz = wk * Akz + wi * Aiz + ws * Asz;
+    const S div = ux + vy + wz;
+    const S two3 = S(2.0 / 3.0);
+    const S s11 = (S(2) * ux - two3 * div) * Amu;
+    const S s22 = (S(2) * vy - two3 * div) * Amu;
+    const S s33 = (S(2) * wz - two3 * div) * Amu;
+    const S s12 = (uy + vx) * Amu;
+    const S s13 = (uz + wx) * Amu;
+    const S s23 = (vz + wy) * Amu;
+    const S Amuk = Amu * S(A.c.vis_c);
+    const S Tx = Tk * Akx + Ti * Aix + Ts * Asx;
+    const S Ty = Tk * Aky + Ti * Aiy + Ts * Asy;
+    const S Tz = Tk * Akz + Ti * Aiz + Ts * Asz;
+    const S u = q[0][4], v = q[1][4], w = q[2][4];
+    const S E1 = u * s11 + v * s12 + w * s13 + Amuk * Tx;
+    const S E2 = u * s12 + v * s22 + w * s23 + Amuk * Ty;
+    const S E3 = u * s13 + v * s23 + w * s33 + Amuk * Tz;
+    const S Aj1 = S(A.M[M_JAC][qi]);
+    const S ax[3] = {Akx, Aix, Asx}, ay[3] = {Aky, Aiy, Asy}, az[3] = {Akz, Aiz, Asz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const S akx = ax[d] * Aj1, aky = ay[d] * Aj1, akz = az[d] * Aj1;
+        S* E = A.ev + (long long)(4 * d) * fs + qi;
+        E[0] = (akx * s11 + aky * s12 + akz * s13);
+        E[fs] = (akx * s12 + aky * s22 + akz * s23);
+        E[2 * fs] = (akx * s13 + aky * s23 + akz * s33);
+        E[3 * fs] = (akx * E1 + aky * E2 + akz * E3);
+    }
+    
+}
+
 // The plane loads of both kernels go through per-thread slots: the slot's
 // LDS position and its global offset in plane 0 (-1: outside the ghost
 // layers, stored as 0).  Loads for plane k+1 are issued before plane k is
@@ -186,66 +255,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
             for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(k + 5) * g.sz];
             load_plane(k + 1);
         }
-        if (t.own) {
-            S gr[4][3];
-#pragma unroll
-            for (int f = 0; f < 4; f++) {
-                S s[9];
-#pragma unroll
-                for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + 4) * PW + tx + m];
-                gr[f][0] = cd9<SV>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
-#pragma unroll
-                for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + m) * PW + tx + 4];
-                gr[f][1] = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
-                gr[f][2] = cd9<SV>(q[f], k, g.nz, A.bnd[4], A.bnd[5], rhz);
-            }
-            // vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker
-            const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
-            const S Akx = S(A.M[M_KX][qi]), Aky = S(A.M[M_KY][qi]), Akz = S(A.M[M_KZ][qi]);
-            const S Aix = S(A.M[M_IX][qi]), Aiy = S(A.M[M_IY][qi]), Aiz = S(A.M[M_IZ][qi]);
-            const S Asx = S(A.M[M_SX][qi]), Asy = S(A.M[M_SY][qi]), Asz = S(A.M[M_SZ][qi]);
-            const S Amu = A.amu[qi];
-            const S uk = gr[0][0], ui = gr[0][1], us = gr[0][2];
-            const S vk = gr[1][0], vi = gr[1][1], vs = gr[1][2];
-            const S wk = gr[2][0], wi = gr[2][1], ws = gr[2][2];
-            const S Tk = gr[3][0], Ti = gr[3][1], Ts = gr[3][2];
-            const S ux = uk * Akx + ui * Aix + us * Asx;
-            const S vx = vk * Akx + vi * Aix + vs * Asx;
-            const S wx = wk * Akx + wi * Aix + ws * Asx;
-            const S uy = uk * Aky + ui * Aiy + us * Asy;
-            const S vy = vk * Aky + vi * Aiy + vs * Asy;
-            const S wy = wk * Aky + wi * Aiy + ws * Asy;
-            const S uz = uk * Akz + ui * Aiz + us * Asz;
-            const S vz = vk * Akz + vi * Aiz + vs * Asz;
-            const S wz = wk * Akz + wi * Aiz + ws * Asz;
-            const S div = ux + vy + wz;
-            const S two3 = S(2.0 / 3.0);
-            const S s11 = (S(2) * ux - two3 * div) * Amu;
-            const S s22 = (S(2) * vy - two3 * div) * Amu;
-            const S s33 = (S(2) * wz - two3 * div) * Amu;
-            const S s12 = (uy + vx) * Amu;
-            const S s13 = (uz + wx) * Amu;
-            const S s23 = (vz + wy) * Amu;
-            const S Amuk = Amu * S(A.c.vis_c);
-            const S Tx = Tk * Akx + Ti * Aix + Ts * Asx;
-            const S Ty = Tk * Aky + Ti * Aiy + Ts * Asy;
-            const S Tz = Tk * Akz + Ti * Aiz + Ts * Asz;
-            const S u = q[0][4], v = q[1][4], w = q[2][4];
-            const S E1 = u * s11 + v * s12 + w * s13 + Amuk * Tx;
-            const S E2 = u * s12 + v * s22 + w * s23 + Amuk * Ty;
-            const S E3 = u * s13 + v * s23 + w * s33 + Amuk * Tz;
-            const S Aj1 = S(A.M[M_JAC][qi]);
-            const S ax[3] = {Akx, Aix, Asx}, ay[3] = {Aky, Aiy, Asy}, az[3] = {Akz, Aiz, Asz};
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const S akx = ax[d] * Aj1, aky = ay[d] * Aj1, akz = az[d] * Aj1;
-                S* E = A.ev + (long long)(4 * d) * fs + qi;
-                E[0] = (akx * s11 + aky * s12 + akz * s13);
-                E[fs] = (akx * s12 + aky * s22 + akz * s23);
-                E[2 * fs] = (akx * s13 + aky * s23 + akz * s33);
-                E[3 * fs] = (akx * E1 + aky * E2 + akz * E3);
-            }
-        }
+        if (t.own) gf_point<SV, S, PW>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
 #pragma unroll
         for (int f = 0; f < 4; f++)
 #pragma unroll
