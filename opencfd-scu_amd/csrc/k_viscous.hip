@@ -27,6 +27,8 @@
 // the FP32 path; divergence: 16).  x and y neighbours come from the
 // current plane staged in LDS with its 4-point halo; z neighbours come from a
 // 9-plane queue each thread keeps in registers for its own column.
+#include <type_traits>
+
 #include "ocfd_dev.h"
 #include "ocfd_kernels.h"
 
@@ -67,17 +69,19 @@ constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
 // Central derivative at interior index i (0..n-1) from s[0..8] = values at
 // i-4..i+4 (OCFD_CD6/CD8_kernel + D0 closures + put_d0_kernel).  rh = 1/h; the
 // reference's divisions by 12, 60, 840 and h are multiplications by the
-// reciprocals here (a difference of an ulp or two per derivative).
-template <int SV, typename R>
+// reciprocals here (a difference of an ulp or two per derivative).  CL =
+// false compiles the interior formula alone, for callers that know that no
+// closure is in reach.
+template <int SV, typename R, bool CL = true>
 __device__ __forceinline__ R cd9(const R* s, int i, int n, int bl, int br, R rh) {
     const R* c = s + 4;
     R tmp;
-    if (bl && i <= 3) {
+    if (CL && bl && i <= 3) {
         if (i == 0) tmp = (c[1] - c[0]);
         else if (i == 1) tmp = (c[1] - c[-1]) * R(0.5);
         else if (i == 2) tmp = (c[-2] - R(8) * c[-1] + R(8) * c[1] - c[2]) * R(1.0 / 12.0);
         else tmp = (c[3] - c[-3] - R(9) * (c[2] - c[-2]) + R(45) * (c[1] - c[-1])) * R(1.0 / 60.0);
-    } else if (br && i >= n - 4) {
+    } else if (CL && br && i >= n - 4) {
         const int t1 = i - n;
         if (t1 == -1) tmp = (c[0] - c[-1]);
         else if (t1 == -2) tmp = (c[1] - c[-1]) * R(0.5);
@@ -115,29 +119,34 @@ __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
 // Gradients of u, v, w, T at (t.i, t.j, k) from the plane tile pl (the
 // current plane with its x and y strips, row pitch PW) and the thread's z
 // queue q, and at the same point the viscous fluxes Ev1..4 of the three
-// directions (vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker).
-template <int SV, typename S, int PW>
+// directions (vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker).  CL =
+// false: no derivative of the block's plane reaches a closure (the caller
+// decides per block and plane), and the body is straight-line code.
+template <int SV, typename S, int PW, bool CL>
 __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k, const S (*pl)[PW * (TY + 8)],
                                          const S (*q)[9], int tx, int ty, S rhx, S rhy, S rhz) {
     const Geo& g = A.g;
+    // the point's metrics and viscosity first: their loads are in flight
+    // while the derivatives are formed
+    const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
+    const S Akx = S(A.M[M_KX][qi]), Aky = S(A.M[M_KY][qi]), Akz = S(A.M[M_KZ][qi]);
+    const S Aix = S(A.M[M_IX][qi]), Aiy = S(A.M[M_IY][qi]), Aiz = S(A.M[M_IZ][qi]);
+    const S Asx = S(A.M[M_SX][qi]), Asy = S(A.M[M_SY][qi]), Asz = S(A.M[M_SZ][qi]);
+    const S Amu = A.amu[qi];
+    const S Aj1 = S(A.M[M_JAC][qi]);
     S gr[4][3];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
         S s[9];
 #pragma unroll
         for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + 4) * PW + tx + m];
-        gr[f][0] = cd9<SV>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
+        gr[f][0] = cd9<SV, S, CL>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
 #pragma unroll
         for (int m = 0; m < 9; m++) s[m] = pl[f][(ty + m) * PW + tx + 4];
-        gr[f][1] = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
-        gr[f][2] = cd9<SV>(q[f], k, g.nz, A.bnd[4], A.bnd[5], rhz);
+        gr[f][1] = cd9<SV, S, CL>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
+        gr[f][2] = cd9<SV, S, CL>(q[f], k, g.nz, A.bnd[4], A.bnd[5], rhz);
     }
     // vis_flux_s_ker / vis_flux_e_ker / vis_flus_ev_ker
-    const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
-    const S Akx = S(A.M[M_KX][qi]), Aky = S(A.M[M_KY][qi]), Akz = S(A.M[M_KZ][qi]);
-    const S Aix = S(A.M[M_IX][qi]), Aiy = S(A.M[M_IY][qi]), Aiz = S(A.M[M_IZ][qi]);
-    const S Asx = S(A.M[M_SX][qi]), Asy = S(A.M[M_SY][qi]), Asz = S(A.M[M_SZ][qi]);
-    const S Amu = A.amu[qi];
     const S uk = gr[0][0], ui = gr[0][1], us = gr[0][2];
     const S vk = gr[1][0], vi = gr[1][1], vs = gr[1][2];
     const S wk = gr[2][0], wi = gr[2][1], ws = gr[2][2];
@@ -167,7 +176,6 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
     const S E1 = u * s11 + v * s12 + w * s13 + Amuk * Tx;
     const S E2 = u * s12 + v * s22 + w * s23 + Amuk * Ty;
     const S E3 = u * s13 + v * s23 + w * s33 + Amuk * Tz;
-    const S Aj1 = S(A.M[M_JAC][qi]);
     const S ax[3] = {Akx, Aix, Asx}, ay[3] = {Aky, Aiy, Asy}, az[3] = {Akz, Aiz, Asz};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -179,6 +187,15 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
         E[3 * fs] = (akx * E1 + aky * E2 + akz * E3);
     }
     
+}
+
+// Whether a central-derivative closure is in reach of the block's plane k:
+// the same for every thread of the block.
+template <typename S>
+__device__ __forceinline__ bool near_closure(const ViscArgsT<S>& A, const Tile25& t, int k) {
+    const Geo& g = A.g;
+    return (A.bnd[0] && t.i0 <= 3) || (A.bnd[1] && t.i0 + TX > g.nx - 4) || (A.bnd[2] && t.j0 <= 3) ||
+           (A.bnd[3] && t.j0 + TY > g.ny - 4) || (A.bnd[4] && k <= 3) || (A.bnd[5] && k >= g.nz - 4);
 }
 
 // The plane loads of both kernels go through per-thread slots: the slot's
@@ -255,7 +272,10 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
             for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(k + 5) * g.sz];
             load_plane(k + 1);
         }
-        if (t.own) gf_point<SV, S, PW>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+        if (t.own) {
+            if (near_closure(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+            else gf_point<SV, S, PW, false>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+        }
 #pragma unroll
         for (int f = 0; f < 4; f++)
 #pragma unroll
@@ -345,7 +365,9 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
             for (int f = 0; f < NC; f++) nq[f] = EZ[f * fs + col + (long long)(k + 5) * g.sz];
             load_plane(k + 1);
         }
-        if (t.own) {
+        // CL = false: no closure in reach of the block's plane (straight-line code)
+        auto point = [&](auto cl) {
+            constexpr bool CL = decltype(cl)::value;
             const long long qi = gidx(g, t.i, t.j, k);
             const double ajac = A.M[M_JAC][qi];
 #pragma unroll
@@ -353,11 +375,11 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
                 S s[9];
 #pragma unroll
                 for (int r = 0; r < 9; r++) s[r] = ex[m][ty * XW + tx + r];
-                const S vx = cd9<SV>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
+                const S vx = cd9<SV, S, CL>(s, t.i, g.nx, A.bnd[0], A.bnd[1], rhx);
 #pragma unroll
                 for (int r = 0; r < 9; r++) s[r] = ey[m][(ty + r) * TX + tx];
-                const S vy = cd9<SV>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
-                const S vz = cd9<SV>(q[m], k, g.nz, A.bnd[4], A.bnd[5], rhz);
+                const S vy = cd9<SV, S, CL>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
+                const S vz = cd9<SV, S, CL>(q[m], k, g.nz, A.bnd[4], A.bnd[5], rhz);
                 // the Jacobian factor and the sums are double
                 if (SEP) {
                     double acc = ajac * double(vx);
@@ -373,6 +395,10 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
                     *du = S(acc);
                 }
             }
+        };
+        if (t.own) {
+            if (near_closure(A, t, k)) point(std::true_type{});
+            else point(std::false_type{});
         }
 #pragma unroll
         for (int f = 0; f < NC; f++)
