@@ -368,28 +368,74 @@ __global__ void k_mirror_y(Geo g, T* f, double sign) {
 // (dp>i2).  Node (i,j,k) with i,j,k >= -1 serves the x flags when j,k >= 0,
 // the y flags when i,k >= 0 and the z flags when i,j >= 0.
 // ---------------------------------------------------------------------------
+#ifndef OCFD_SENSOR_KZ
+#define OCFD_SENSOR_KZ 16  // nodes along z per block of the sensor
+#endif
+constexpr int SENSOR_KZ = OCFD_SENSOR_KZ;
+// A block owns a 64 x 4 tile of (i, j) nodes and marches through SENSOR_KZ
+// planes: p of the tile's plane k and its one-node rim is staged in LDS (x
+// and y neighbours), each column keeps p at k-1, k, k+1 in registers (z
+// neighbours), so every p is formed once per block instead of seven times.
+// Same arithmetic as p00*rho*T and the dp sum in x, y, z order.
 template <typename S>
 __global__ void __launch_bounds__(256) k_sensor(Geo g, Consts c, const S* __restrict__ d,
                                                 const S* __restrict__ T, int8_t* fx, int8_t* fy, int8_t* fz) {
+    constexpr int W = 66;                      // tile row with its rim
+    __shared__ double pl[6 * W];               // rows j0-1 .. j0+4, columns i0-1 .. i0+64
     const Blk B = xcd_block();
-    const int i = B.x * 64 + threadIdx.x - 1, j = B.y * 4 + threadIdx.y - 1, k = B.z - 1;
-    if (i >= g.nx || j >= g.ny) return;
-    const long long q = gidx(g, i, j, k);
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+    const int i0 = B.x * 64 - 1, j0 = B.y * 4 - 1;
+    const int i = i0 + tx, j = j0 + ty;
+    const int kb = B.z * SENSOR_KZ - 1, ke = min(kb + SENSOR_KZ, g.nz);
+    const bool own = i < g.nx && j < g.ny;
     auto P = [&](long long o) { return c.p00 * double(d[o]) * double(T[o]); };
-    const double pc = P(q);
-    const long long sx = 1, sy = g.ldx, sz = g.sz;
-    double pm = P(q - sx), pp = P(q + sx);
-    double dp0 = fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
-    pm = P(q - sy); pp = P(q + sy);
-    dp0 += fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
-    pm = P(q - sz); pp = P(q + sz);
-    dp0 += fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
-    int kp = 1;
-    if (dp0 > c.hy_i1) kp += 1;
-    if (dp0 > c.hy_i2) kp += 1;
-    if (j >= 0 && k >= 0) fx[q] = (int8_t)kp;
-    if (i >= 0 && k >= 0) fy[q] = (int8_t)kp;
-    if (i >= 0 && j >= 0) fz[q] = (int8_t)kp;
+    // the lane's own column: its real position inside the LAP box (an own
+    // lane's neighbours are at most nx, ny), else any valid column (unused)
+    const long long col = (i <= g.nx + 3 && j <= g.ny + 3) ? gidx(g, i, j, 0) : gidx(g, 0, 0, 0);
+    // rim slots: lanes 0..131 the rows j0-1 and j0+4 (66 columns each),
+    // lanes 132..139 the columns i0-1 and i0+64 of the 4 tile rows
+    int hs = -1;
+    long long ho = -1;
+    if (tid < 2 * W) {
+        const int r = tid / W, xx = tid % W, gi = i0 - 1 + xx, gj = r ? j0 + 4 : j0 - 1;
+        hs = (r ? 5 : 0) * W + xx;
+        if (gi <= g.nx + 3 && gj <= g.ny + 3) ho = gidx(g, gi, gj, 0);
+    } else if (tid < 2 * W + 8) {
+        const int r = tid - 2 * W, side = r >> 2, row = r & 3, gi = side ? i0 + 64 : i0 - 1, gj = j0 + row;
+        hs = (row + 1) * W + (side ? W - 1 : 0);
+        if (gi <= g.nx + 3 && gj <= g.ny + 3) ho = gidx(g, gi, gj, 0);
+    }
+    if (kb >= ke) return;
+    double pm = P(col + (long long)(kb - 1) * g.sz), pc = P(col + (long long)kb * g.sz);
+    double pp = P(col + (long long)(kb + 1) * g.sz);
+    double hv = ho >= 0 ? P(ho + (long long)kb * g.sz) : 0.0;
+    for (int k = kb; k < ke; k++) {
+        pl[(ty + 1) * W + tx + 1] = pc;
+        if (hs >= 0) pl[hs] = hv;
+        __syncthreads();
+        // the next plane's values are in flight while this one is decided
+        double pn = 0.0, hn = 0.0;
+        if (k + 1 < ke) {
+            pn = P(col + (long long)(k + 2) * g.sz);
+            hn = ho >= 0 ? P(ho + (long long)(k + 1) * g.sz) : 0.0;
+        }
+        if (own) {
+            const long long q = gidx(g, i, j, k);
+            double am = pl[(ty + 1) * W + tx], ap = pl[(ty + 1) * W + tx + 2];
+            double dp0 = fabs(-am + 2 * pc - ap) / (am + 2 * pc + ap);
+            am = pl[ty * W + tx + 1]; ap = pl[(ty + 2) * W + tx + 1];
+            dp0 += fabs(-am + 2 * pc - ap) / (am + 2 * pc + ap);
+            dp0 += fabs(-pm + 2 * pc - pp) / (pm + 2 * pc + pp);
+            int kp = 1;
+            if (dp0 > c.hy_i1) kp += 1;
+            if (dp0 > c.hy_i2) kp += 1;
+            if (j >= 0 && k >= 0) fx[q] = (int8_t)kp;
+            if (i >= 0 && k >= 0) fy[q] = (int8_t)kp;
+            if (i >= 0 && j >= 0) fz[q] = (int8_t)kp;
+        }
+        __syncthreads();
+        pm = pc; pc = pp; pp = pn; hv = hn;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -684,7 +730,8 @@ void launch_face_refresh(const BCArgsT<S>& a, hipStream_t st) {
 template <typename S>
 void launch_sensor(const Geo& g, const Consts& c, const S* d, const S* T, int8_t* fx, int8_t* fy, int8_t* fz,
                    hipStream_t st) {
-    dim3 grid((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    // nodes i, j, k = -1 .. n-1
+    dim3 grid((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, (g.nz + 1 + SENSOR_KZ - 1) / SENSOR_KZ);
     hipLaunchKernelGGL(k_sensor<S>, grid, dim3(64, 4), 0, st, g, c, d, T, fx, fy, fz);
 }
 
