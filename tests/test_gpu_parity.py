@@ -94,6 +94,35 @@ def test_hybrid_flag_mixes(intv, closed):
     assert (e <= 1e-4).all(), e
 
 
+@pytest.mark.parametrize("closed", [False, True])
+def test_hybrid_tall_z(closed):
+    """A 20x12x40 wavy box: the sensor's blocks (16 nodes along z each) and
+    the gradient/flux and divergence kernels' blocks (32 and 16 planes on the
+    FP64 path, 8 and 16 on the FP32 path) meet inside the grid, with and
+    without closures on every face; the oracle's flags bit for bit and its du
+    to the bar."""
+    case = cases.wavy_box(20, 12, 40, scheme="SCHEME_HYBRIDAUTO", amp=0.08,
+                          periodic=(0, 0, 0) if closed else (1, 1, 1))
+    case.cfg.hy_dp_intv[0], case.cfg.hy_dp_intv[1] = 0.005, 0.02
+    if closed:
+        for q in range(6):
+            case.cfg.dif_bound[q] = 1
+    g, o = pair(case)
+    g.rhs(1)
+    o.rhs(1)
+    for a, b in zip(g.get_flags(), o.get_flags()):
+        assert np.array_equal(a, b)
+    e = relerr(g.get_du(), o.get_du())
+    report(f"rhs hybrid tall z closed={closed}", e)
+    assert (e <= TOL).all(), e
+    case.cfg.precision = capi.PREC_MIXED
+    m = case.load(capi.Solver(case.cfg))
+    m.rhs(1)
+    e = relerr(m.get_du(), o.get_du())
+    report(f"rhs hybrid tall z mixed closed={closed}", e)
+    assert (e <= 1e-4).all(), e
+
+
 @pytest.mark.parametrize("kw", [{}, {"mzmax": 3, "tw": -1.0, "upper": 1, "withleading": 1},
                                 {"mzmax": -2, "scheme": "SCHEME_HYBRIDAUTO"}])
 def test_liftbody_ibc124_steps(kw):
