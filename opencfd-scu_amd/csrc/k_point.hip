@@ -46,12 +46,13 @@ __global__ void __launch_bounds__(256) k_stage(StageArgsT<S> A) {
     if (i >= A.g.nx || j >= A.g.ny) return;
     const long long q = gidx(A.g, i, j, k);
     // du, plus its viscous part dv (components 1..4) when that is kept apart
+    // du and dv are read once, as streaming (non-temporal) loads
     double du[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) du[m] = double(A.du[m * A.fs + q]);
+    for (int m = 0; m < 5; m++) du[m] = double(__builtin_nontemporal_load(&A.du[m * A.fs + q]));
     if (A.dv) {
 #pragma unroll
-        for (int m = 1; m < 5; m++) du[m] += double(A.dv[(m - 1) * A.fs + q]);
+        for (int m = 1; m < 5; m++) du[m] += double(__builtin_nontemporal_load(&A.dv[(m - 1) * A.fs + q]));
     }
     stage_point(A, q, du);
 }

@@ -128,12 +128,17 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
     const Geo& g = A.g;
     // the point's metrics and viscosity first: their loads are in flight
     // while the derivatives are formed
+    // The metrics are read once here and the fluxes are written once: both
+    // go through the caches as streaming (non-temporal) accesses, which
+    // leaves L2 to the plane strips that neighbouring tiles share (measured
+    // 203 -> 197 us, and the divergence that reads Ev next 42 -> 40 us).
     const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
-    const S Akx = S(A.M[M_KX][qi]), Aky = S(A.M[M_KY][qi]), Akz = S(A.M[M_KZ][qi]);
-    const S Aix = S(A.M[M_IX][qi]), Aiy = S(A.M[M_IY][qi]), Aiz = S(A.M[M_IZ][qi]);
-    const S Asx = S(A.M[M_SX][qi]), Asy = S(A.M[M_SY][qi]), Asz = S(A.M[M_SZ][qi]);
+    auto ld = [&](int m) { return S(__builtin_nontemporal_load(&A.M[m][qi])); };
+    const S Akx = ld(M_KX), Aky = ld(M_KY), Akz = ld(M_KZ);
+    const S Aix = ld(M_IX), Aiy = ld(M_IY), Aiz = ld(M_IZ);
+    const S Asx = ld(M_SX), Asy = ld(M_SY), Asz = ld(M_SZ);
     const S Amu = A.amu[qi];
-    const S Aj1 = S(A.M[M_JAC][qi]);
+    const S Aj1 = ld(M_JAC);
     S gr[4][3];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
@@ -181,10 +186,10 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
     for (int d = 0; d < 3; d++) {
         const S akx = ax[d] * Aj1, aky = ay[d] * Aj1, akz = az[d] * Aj1;
         S* E = A.ev + (long long)(4 * d) * fs + qi;
-        E[0] = (akx * s11 + aky * s12 + akz * s13);
-        E[fs] = (akx * s12 + aky * s22 + akz * s23);
-        E[2 * fs] = (akx * s13 + aky * s23 + akz * s33);
-        E[3 * fs] = (akx * E1 + aky * E2 + akz * E3);
+        __builtin_nontemporal_store(S(akx * s11 + aky * s12 + akz * s13), &E[0]);
+        __builtin_nontemporal_store(S(akx * s12 + aky * s22 + akz * s23), &E[fs]);
+        __builtin_nontemporal_store(S(akx * s13 + aky * s23 + akz * s33), &E[2 * fs]);
+        __builtin_nontemporal_store(S(akx * E1 + aky * E2 + akz * E3), &E[3 * fs]);
     }
     
 }

