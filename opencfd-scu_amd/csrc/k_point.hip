@@ -471,9 +471,9 @@ __global__ void __launch_bounds__(256) k_filter(FilterArgsT<S> A) {
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         const double* F = A.flap + (long long)m * A.fs;
-        A.f[(long long)m * A.fs + L] =
-            F[L] - (Sc1 * (c1 * (F[L + st] - F[L]) + c2 * (F[L + 2 * st] - F[L - st])) -
-                    Sc2 * (c1 * (F[L] - F[L - st]) + c2 * (F[L + st] - F[L - 2 * st])));
+        const double r = F[L] - (Sc1 * (c1 * (F[L + st] - F[L]) + c2 * (F[L + 2 * st] - F[L - st])) -
+                                 Sc2 * (c1 * (F[L] - F[L - st]) + c2 * (F[L + st] - F[L - 2 * st])));
+        __builtin_nontemporal_store(r, &A.f[(long long)m * A.fs + L]);  // streamed (125 -> 96 us)
     }
 }
 

@@ -128,16 +128,17 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
     const Geo& g = A.g;
     // the point's metrics and viscosity first: their loads are in flight
     // while the derivatives are formed
-    // The metrics are read once here and the fluxes are written once: both
-    // go through the caches as streaming (non-temporal) accesses, which
-    // leaves L2 to the plane strips that neighbouring tiles share (measured
-    // 203 -> 197 us, and the divergence that reads Ev next 42 -> 40 us).
+    // The metrics and the viscosity are read once here and the fluxes are
+    // written once: all go through the caches as streaming (non-temporal)
+    // accesses, which leaves L2 to the plane strips that neighbouring tiles
+    // share (measured 203 -> 196 us, and the divergence that reads Ev next
+    // 42 -> 40 us).
     const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
     auto ld = [&](int m) { return S(__builtin_nontemporal_load(&A.M[m][qi])); };
     const S Akx = ld(M_KX), Aky = ld(M_KY), Akz = ld(M_KZ);
     const S Aix = ld(M_IX), Aiy = ld(M_IY), Aiz = ld(M_IZ);
     const S Asx = ld(M_SX), Asy = ld(M_SY), Asz = ld(M_SZ);
-    const S Amu = A.amu[qi];
+    const S Amu = __builtin_nontemporal_load(&A.amu[qi]);
     const S Aj1 = ld(M_JAC);
     S gr[4][3];
 #pragma unroll
