@@ -126,13 +126,12 @@ template <int SV, typename S, int PW, bool CL>
 __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k, const S (*pl)[PW * (TY + 8)],
                                          const S (*q)[9], int tx, int ty, S rhx, S rhy, S rhz) {
     const Geo& g = A.g;
-    // the point's metrics and viscosity first: their loads are in flight
-    // while the derivatives are formed
-    // The metrics and the viscosity are read once here and the fluxes are
-    // written once: all go through the caches as streaming (non-temporal)
-    // accesses, which leaves L2 to the plane strips that neighbouring tiles
-    // share (measured 203 -> 196 us, and the divergence that reads Ev next
-    // 42 -> 40 us).
+    // The point's metrics and viscosity are loaded first, so that the loads
+    // are in flight while the derivatives are formed.  They are read once
+    // here and the fluxes are written once: all go through the caches as
+    // streaming (non-temporal) accesses, which leaves L2 to the plane strips
+    // that neighbouring tiles share (measured 203 -> 196 us, and the
+    // divergence that reads Ev next 42 -> 40 us).
     const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
     auto ld = [&](int m) { return S(__builtin_nontemporal_load(&A.M[m][qi])); };
     const S Akx = ld(M_KX), Aky = ld(M_KY), Akz = ld(M_KZ);
