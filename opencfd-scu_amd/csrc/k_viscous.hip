@@ -478,7 +478,11 @@ void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
 #define OCFD_VDIV_NC_D 1  // flux components per divergence launch, FP64 path
 #endif
 template <typename S> struct NC_ { static constexpr int v = OCFD_VDIV_NC_D; };
-template <> struct NC_<float> { static constexpr int v = 4; };
+#ifndef OCFD_VDIV_NC_F
+#define OCFD_VDIV_NC_F 4  // flux components per divergence launch, FP32 path (measured on the
+                          // 512x128x64 ramp: 4 in one launch 113 us, 2 x 61 us, 4 x 34 us)
+#endif
+template <> struct NC_<float> { static constexpr int v = OCFD_VDIV_NC_F; };
 template <typename S>
 int visc_div_components() { return NC_<S>::v; }
 template <typename S>
