@@ -13,11 +13,21 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
-    try:
-        import torch  # noqa: F401  (only to ask whether a device is visible)
-        return torch.cuda.is_available()
-    except Exception:
+    """Whether an AMD GPU is visible, asked of the kernel driver's topology
+    without starting any HIP runtime in this process: the library's runtime
+    must be the first to open the device (a runtime started here first, such
+    as torch's own copy, leaves the library's hipSetDevice failing)."""
+    import glob
+    import os
+    if not os.path.exists("/dev/kfd"):
         return False
+    for p in glob.glob("/sys/class/kfd/kfd/topology/nodes/*/gpu_id"):
+        try:
+            if int(open(p).read().strip() or 0) != 0:
+                return True
+        except (OSError, ValueError):
+            pass
+    return False
 
 
 @pytest.fixture(scope="session")
