@@ -32,6 +32,10 @@ extern "C" {
 #define OCFD_MAX_FILTERS 10 /* src/parameters.c:102 (NF_max) */
 #define OCFD_MAX_ANA 16     /* ANA_EVENT& keys kept (the reference mallocs N_ana) */
 #define OCFD_ANA_NPARA 100  /* ANA_NPARA&/ANA_RPARA& values, src/parameters.c:346-358 */
+/* Wall-disturbance time modes kept: MTMAX (BC_NPARA[1] for IBC 108,
+ * BC_NPARA[4] for IBC 124) above this is refused by ocfd_create
+ * (src/OCFD_boundary_init.c:501-517 sizes its tables to MTMAX). */
+#define OCFD_MTMAX 64
 
 /* Scheme identifiers, src/parameters.c:567-637. */
 enum {
@@ -148,8 +152,10 @@ int ocfd_load_flow(ocfd_ctx *ctx, const double *d, const double *u,
                    const double *v, const double *w, const double *T,
                    int istep, double tt);
 /* Inlet profile for IBC 108: ny rows of (d,u,v,T) as 4*ny doubles laid out
- * [var][j] (flow1d-inlet.dat, src/OCFD_boundary_init.c:376-395).  Optional:
- * without it the uniform (1,1,0,1) profile is used when BC_NPARA[2] != 1. */
+ * [var][j] (flow1d-inlet.dat, src/OCFD_boundary_init.c:376-395).  Required
+ * when BC_NPARA[2] (INLET_BOUNDARY) = 1: ocfd_init then fails with
+ * OCFD_ERR_STATE without it; otherwise optional, and the uniform (1,1,0,1)
+ * profile is used. */
 int ocfd_set_inlet_profile(ocfd_ctx *ctx, const double *prof4xny);
 
 /* IBC 124 boundary tables (bc_user_Liftbody3d_init,

@@ -99,7 +99,7 @@ struct ocfd_ctx {
     std::vector<double> mesh;              // x, y, z (interior, 3 arrays) as loaded
     std::vector<double> inlet;             // [4][ny]
     bool have_inlet = false, have_grid = false, have_flow = false, inited = false;
-    double TM[64] = {}, fait[64] = {};
+    double TM[OCFD_MTMAX] = {}, fait[OCFD_MTMAX] = {};
     int x_begin = 0;
     int istep = 0;
     double tt = 0.0;
@@ -379,7 +379,8 @@ int apply_bc(ocfd_ctx* x) {
         double ht = 0.0;
         const double beta = x->cfg.bc_rpara[4];
         if (beta > 0.) {
-            for (int m = 0; m < x->cfg.bc_npara[1]; m++) ht = ht + x->TM[m] * std::sin((m + 1) * beta * x->tt);
+            for (int m = 0; m < x->cfg.bc_npara[1] && m < OCFD_MTMAX; m++)
+                ht = ht + x->TM[m] * std::sin((m + 1) * beta * x->tt);
         } else {
             ht = 1.;
         }
@@ -397,7 +398,8 @@ int apply_bc(ocfd_ctx* x) {
         double ht = 0.0;
         const double beta = x->cfg.bc_rpara[4];
         if (beta > 0.) {
-            for (int m = 0; m < x->cfg.bc_npara[4] && m < 64; m++) ht = ht + x->TM[m] * std::sin((m + 1) * beta * x->tt);
+            for (int m = 0; m < x->cfg.bc_npara[4] && m < OCFD_MTMAX; m++)
+                ht = ht + x->TM[m] * std::sin((m + 1) * beta * x->tt);
         } else {
             ht = 1.;
         }
@@ -593,11 +595,16 @@ int ibc108_init(ocfd_ctx* x) {
     const Geo& g = x->g;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     if (!x->have_inlet) {
+        // INLET_BOUNDARY = 1 reads the profile from flow1d-inlet.dat
+        // (src/OCFD_boundary_init.c:376-395); without it the inlet would be
+        // all zero (d = 0) and the run would turn to NaN
+        if (C.bc_npara[2] == 1)
+            return fail(OCFD_ERR_STATE, "IBC 108 with INLET_BOUNDARY = 1 needs the inlet profile "
+                                        "(ocfd_set_inlet_profile / flow1d-inlet.dat) before ocfd_init");
         x->inlet.assign(4 * ny, 0.0);
-        if (C.bc_npara[2] != 1)
-            for (int j = 0; j < ny; j++) {
-                x->inlet[j] = 1.; x->inlet[ny + j] = 1.; x->inlet[2 * ny + j] = 0.; x->inlet[3 * ny + j] = 1.;
-            }
+        for (int j = 0; j < ny; j++) {
+            x->inlet[j] = 1.; x->inlet[ny + j] = 1.; x->inlet[2 * ny + j] = 0.; x->inlet[3 * ny + j] = 1.;
+        }
     }
     const int MZMAX = C.bc_npara[0], MTMAX = C.bc_npara[1];
     const double XDB = C.bc_rpara[2], XDE = C.bc_rpara[3], XUB = C.bc_rpara[6], SLZ = C.bc_rpara[7];
@@ -637,13 +644,13 @@ int ibc108_init(ocfd_ctx* x) {
         }
     }
     double Ttmp = 0.;
-    for (int k = 0; k < MTMAX && k < 64; k++) {
+    for (int k = 0; k < MTMAX && k < OCFD_MTMAX; k++) {
         x->fait[k] = rand() / (double)RAND_MAX;
         x->TM[0] = 1.;
         if (k > 0) x->TM[k] = x->TM[k - 1] / 1.25;
         Ttmp = Ttmp + x->TM[k];
     }
-    for (int k = 0; k < MTMAX && k < 64; k++) x->TM[k] = x->TM[k] / Ttmp;
+    for (int k = 0; k < MTMAX && k < OCFD_MTMAX; k++) x->TM[k] = x->TM[k] / Ttmp;
     int rc = 0;
     if (!x->ub1) {
         x->ub1 = (double*)alloc_bytes(x, sizeof(double) * (4 * ny + nx + nz), rc);
@@ -778,13 +785,13 @@ int ibc124_init(ocfd_ctx* x) {
     x->v_dist_need = need;
     // get_fait_multifrequancy (:501-517)
     double Ttmp = 0.;
-    for (int k = 0; k < MTMAX && k < 64; k++) {
+    for (int k = 0; k < MTMAX && k < OCFD_MTMAX; k++) {
         x->fait[k] = rand() / (double)RAND_MAX;
         x->TM[0] = 1.;
         if (k > 0) x->TM[k] = x->TM[k - 1] / 1.25;
         Ttmp = Ttmp + x->TM[k];
     }
-    for (int k = 0; k < MTMAX && k < 64; k++) x->TM[k] = x->TM[k] / Ttmp;
+    for (int k = 0; k < MTMAX && k < OCFD_MTMAX; k++) x->TM[k] = x->TM[k] / Ttmp;
     int rc = 0;
     if (!x->b124) {
         x->b124 = (double*)alloc_bytes(x, sizeof(double) * tab.size(), rc);
@@ -856,6 +863,11 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     // INIT_STAT = 0 runs the reference's built-in test geometry and a
     // different boundary routine (bc_user_Liftbody3d_simple), not provided
     if (C.ibc == 124 && C.init_stat != 1) return fail(OCFD_ERR_UNSUPPORTED, "IBC 124 needs INIT_STAT = 1");
+    // the wall-disturbance time tables TM/fait hold OCFD_MTMAX modes
+    // (MTMAX is BC_NPARA[1] for IBC 108, BC_NPARA[4] for IBC 124; the
+    // reference sizes them to MTMAX)
+    if ((C.ibc == 108 && C.bc_npara[1] > OCFD_MTMAX) || (C.ibc == 124 && C.bc_npara[4] > OCFD_MTMAX))
+        return fail(OCFD_ERR_CONFIG, "MTMAX (BC_NPARA) above " + std::to_string(OCFD_MTMAX) + " is not supported");
     if (C.precision != OCFD_PREC_FP64 && C.precision != OCFD_PREC_MIXED)
         return fail(OCFD_ERR_CONFIG, "PRECISION: FP64 or MIXED");
     const long long npts = (long long)(C.nx + 8) * (C.ny + 8) * (C.nz + 8);
@@ -1121,35 +1133,61 @@ bool steps_replayable(const ocfd_ctx* x) {
 // the capture runs the host side of two steps (swaps, counters) and records
 // their launches, the first replay executes them, each further replay
 // advances the counters as two steps do.  Returns the steps done (even).
+// If the capture, its instantiation or the first replay fails, the host
+// state (f/fn, step counter, time) is put back as it was, graphs are turned
+// off for this context and done = 0, so the caller runs the steps eagerly.
 int step_graph(ocfd_ctx* x, int n, int& done) {
     done = 0;
+    double* const f0 = x->f;
+    double* const fn0 = x->fn;
+    const int istep0 = x->istep;
+    const double tt0 = x->tt;
+    auto give_up = [&] {
+        x->f = f0;
+        x->fn = fn0;
+        x->istep = istep0;
+        x->tt = tt0;
+        x->graphs = false;
+        (void)hipGetLastError();
+        return 0;
+    };
     hipGraph_t gr = nullptr;
     hipGraphExec_t ge = nullptr;
-    HIPCK(hipStreamBeginCapture(x->st, hipStreamCaptureModeRelaxed));
+    if (hipStreamBeginCapture(x->st, hipStreamCaptureModeRelaxed) != hipSuccess) return give_up();
     int rc = one_step(x);
     if (!rc) rc = one_step(x);
     const hipError_t ec = hipStreamEndCapture(x->st, &gr);
     if (rc || ec != hipSuccess) {
         if (gr) (void)hipGraphDestroy(gr);
-        return rc ? rc : fail(OCFD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        return give_up();
     }
     if (hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0) != hipSuccess) {
         (void)hipGraphDestroy(gr);
-        return fail(OCFD_ERR_HIP, "hipGraphInstantiate failed");
+        return give_up();
     }
     hipError_t e = hipSuccess;
-    for (; done + 2 <= n && e == hipSuccess; done += 2) {
+    for (; done + 2 <= n; done += 2) {
+        const int istep_before = x->istep;
+        const double tt_before = x->tt;
         if (done > 0) {  // the host side of two more steps (the swaps cancel)
             x->istep += 2;
             x->tt += x->cfg.dt;
             x->tt += x->cfg.dt;
         }
         e = hipGraphLaunch(ge, x->st);
+        if (e != hipSuccess) {
+            x->istep = istep_before;
+            x->tt = tt_before;
+            break;
+        }
     }
     (void)hipStreamSynchronize(x->st);
     (void)hipGraphExecDestroy(ge);
     (void)hipGraphDestroy(gr);
-    if (e != hipSuccess) return fail(OCFD_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        if (done == 0) return give_up();
+        return fail(OCFD_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    }
     return 0;
 }
 

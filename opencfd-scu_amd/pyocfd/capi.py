@@ -264,6 +264,11 @@ def read_records(path, nx, ny, nz, nfield):
 MLP_NPARAM = 1889
 
 
+# Learned-sensor weights shipped with the package (trained by
+# pyocfd.train_sensor).
+SENSOR_PARAMS = Path(__file__).resolve().parent / "sensor_params.npy"
+
+
 def mlp_pack(p) -> np.ndarray:
     """Dict W1 b1 W2 b2 W3 b3 -> the 1889-float parameter vector."""
     return np.concatenate([np.ravel(p[k]) for k in ("W1", "b1", "W2", "b2", "W3", "b3")]).astype(np.float32)

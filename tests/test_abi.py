@@ -145,3 +145,18 @@ def test_create_without_gpu_fails_loudly(hiplib):
     h = C.c_void_p()
     rc = hiplib.ocfd_create(C.byref(cfg), 0, C.byref(h))
     assert rc != 0
+
+
+def test_create_refuses_more_wall_modes_than_the_tables_hold(hiplib):
+    """MTMAX above OCFD_MTMAX (64) is a configuration error, for IBC 108
+    (BC_NPARA[1]) and IBC 124 (BC_NPARA[4]); the check comes before any
+    device call, so it holds on a host without a GPU too."""
+    c = cases.ramp(64, 32, 8, nsteps=3).cfg
+    c.bc_npara[1] = 65
+    h = C.c_void_p()
+    assert hiplib.ocfd_create(C.byref(c), 0, C.byref(h)) == 2  # OCFD_ERR_CONFIG
+    assert b"MTMAX" in hiplib.ocfd_last_error()
+    c = cases.liftbody(nsteps=1).cfg
+    c.bc_npara[4] = 65
+    assert hiplib.ocfd_create(C.byref(c), 0, C.byref(h)) == 2
+    assert b"MTMAX" in hiplib.ocfd_last_error()
