@@ -2035,12 +2035,16 @@ void init(void) {
         int32_t s = -1;
         if (std::fread(&s, 4, 1, fh) == 1 && s >= 0) std::snprintf(name, sizeof name, "OCFD%08d.dat", s);
         std::fclose(fh);
+    } else {
+        std::printf("Opencfd.msg is not exist, read initial file : opencfd.dat ......\n");
     }
+    std::printf("read initial data file: %s \n\n", name);
     std::vector<double> p((size_t)5 * n);
     int istep = 0;
     double tt = 0;
     SHIM(ocfd_read_flow_file(name, g.nx, g.ny, g.nz, p.data(), p.data() + n, p.data() + 2 * n, p.data() + 3 * n,
                              p.data() + 4 * n, &istep, &tt));
+    std::printf("Istep=%d , tt=%lf\n", istep, tt);
     SHIM(ocfd_load_flow(g_ctx, p.data(), p.data() + n, p.data() + 2 * n, p.data() + 3 * n, p.data() + 4 * n, istep,
                         tt));
     if (g_cfg.ibc == 108 && g_cfg.bc_npara[2] == 1) {

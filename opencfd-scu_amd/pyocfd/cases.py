@@ -395,6 +395,83 @@ def ramp(nx=512, ny=128, nz=64, ma=2.9, re=5000.0, angle=24.0, sweep=0.0, delta=
 
 
 # --------------------------------------------------------------------------
+# The reference's benchmark deck (benchmark_comp/opencfd-scu.in)
+# --------------------------------------------------------------------------
+def _exp_stretch(n, first):
+    """s_j in [0,1] with s_0 = 0, s_1 = first: s = (e^{b j/(n-1)} - 1)/(e^b - 1),
+    b by bisection.  This is the wall-normal distribution the deck's inlet
+    profile (flow1d-inlet.dat, first cell 0.01 of a height of 35) is given on."""
+    j = np.arange(n) / (n - 1.0)
+    lo, hi = 1e-6, 60.0
+    for _ in range(200):
+        b = 0.5 * (lo + hi)
+        if math.expm1(b / (n - 1.0)) / math.expm1(b) > first:
+            lo = b
+        else:
+            hi = b
+    b = 0.5 * (lo + hi)
+    return np.expm1(b * j) / math.expm1(b)
+
+
+def deck_ramp(deck, inlet_file, nx_inlet=12, nx_corner=12, l_inlet=300.0, l_corner=50.0, buff_ratio=1.01,
+              angle=24.0, height=35.0, first=0.01, span=14.0) -> Case:
+    """Mesh and initial field for the reference's benchmark deck (its
+    opencfd-scu.in, read by this library's own parser, and its
+    flow1d-inlet.dat).  The reference builds them with separate generators
+    (benchmark_comp/swept-compression-ramp.c with swept-compression-grid.in,
+    benchmark_comp/init3d-inlet1d.c); this is this project's generator for the
+    same kind of grid, from the grid file's parameters (the defaults):
+
+    * wall line: a flat plate from x = -(l_inlet + l_corner) (nx_inlet
+      uniform points), then nx_corner points of spacing 2 l_corner /
+      (nx_corner - 1) through the corner at x = 0, where the wall turns up by
+      `angle`, then the remaining points continuing the ramp with the spacing
+      growing by buff_ratio per cell;
+    * grid lines leave the wall along a normal that turns from vertical to the
+      ramp normal across the corner points, `height` long, with the
+      exponential distribution of first cell `first` (at the flat plate it
+      reproduces the y column of flow1d-inlet.dat);
+    * z uniform over [0, span] (z is periodic in the deck).
+
+    The initial field is the inlet profile at every x and z, w = 0 (what
+    init3d-inlet1d.c writes to its flow3d0.dat)."""
+    cfg = capi.read_config(deck)
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    prof = np.loadtxt(inlet_file, skiprows=1)[:ny]
+    th = math.radians(angle)
+    h0 = 2.0 * l_corner / (nx_corner - 1)
+    xw, yw, phi = [], [], []
+    for i in range(nx_inlet):
+        xw.append(-(l_inlet + l_corner) + i * (l_inlet - h0) / (nx_inlet - 1))
+        yw.append(0.0)
+        phi.append(0.0)
+    for i in range(nx_corner):
+        s = -l_corner + i * h0
+        xw.append(s if s < 0 else s * math.cos(th))
+        yw.append(0.0 if s < 0 else s * math.sin(th))
+        phi.append(th * i / (nx_corner - 1))
+    ds = h0
+    while len(xw) < nx:
+        ds *= buff_ratio
+        xw.append(xw[-1] + ds * math.cos(th))
+        yw.append(yw[-1] + ds * math.sin(th))
+        phi.append(th)
+    xw, yw, phi = np.array(xw[:nx]), np.array(yw[:nx]), np.array(phi[:nx])
+    sj = _exp_stretch(ny, first / height)
+    X2 = xw[None, :] - height * np.sin(phi)[None, :] * sj[:, None]
+    Y2 = yw[None, :] + height * np.cos(phi)[None, :] * sj[:, None]
+    zs = np.arange(nz) * (span / (nz - 1))
+    x = np.broadcast_to(X2, (nz, ny, nx)).copy()
+    y = np.broadcast_to(Y2, (nz, ny, nx)).copy()
+    z = np.broadcast_to(zs[:, None, None], (nz, ny, nx)).copy()
+    d, u, v, T = (np.broadcast_to(prof[:, c][None, :, None], (nz, ny, nx)).copy() for c in (1, 2, 3, 4))
+    w = np.zeros((nz, ny, nx))
+    inlet = np.stack([prof[:, 1], prof[:, 2], prof[:, 3], prof[:, 4]])
+    return Case("deck%dx%dx%d" % (nx, ny, nz), cfg, [d, u, v, w, T], mesh=(x, y, z), inlet=inlet,
+                extra={"inlet_y": prof[:, 0]})
+
+
+# --------------------------------------------------------------------------
 # Lifting body / body of revolution (IBC 124)
 # --------------------------------------------------------------------------
 def liftbody(nx=48, ny=24, nz=20, ma=2.0, re=2000.0, aoa=2.0, tw=1.5, delta=0.15, r_out=2.0, cfl=0.4,

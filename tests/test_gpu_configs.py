@@ -1,0 +1,255 @@
+"""The five benchmark workloads on the GPU against the CPU oracle, at their
+own sizes where the oracle can follow, on reduced grids of the same
+generator otherwise.
+
+  C1  Taylor-Green vortex 64^3, WENO7 + CD6, periodic: 200 steps (the
+      workload's full length) against the oracle.
+  C2  ramp 512x128x64, hybrid_auto + CD8, IBC 108, filter every 10: 20 steps
+      at full size against the oracle.
+  C3  shock-turbulence interaction, WENO7: 10 steps on 64x32x32 against the
+      oracle (FP64, and the mixed path's L2 and max error against the FP64
+      oracle); one right-hand side at the full 512x256x256 size.
+  C4  learned sensor on the ramp: inference on a small ramp against the NumPy
+      restatement (oracle/mlp_sensor.py) with the oracle's own LAP fields,
+      ghost layers included; the full 1024x256x128 grid with the shipped
+      weights.
+  C5  swept ramp (30 degrees): 10 steps on 64x40x16 against the oracle with
+      the hybrid flags bit for bit; one right-hand side at the full
+      1024x384x256 size, then steps with property checks.
+
+Bars: FP64 relative max-norm per conserved variable 1e-12 (the plain
+target), flags bit-exact; mixed path: relative L2 per variable 1e-6 and
+relative max 1e-4 against the FP64 oracle.  The GPU and the oracle differ
+only by floating-point contraction (fused multiply-adds on the GPU).
+"""
+import gc
+
+import numpy as np
+import pytest
+
+import mlp_sensor as M
+import oracle as O
+from pyocfd import capi, cases
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def relerr(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    return np.array([np.abs(a[v] - b[v]).max() / max(np.abs(b[v]).max(), 1e-300) for v in range(a.shape[0])])
+
+
+def rel_l2(a, b):
+    a = np.asarray(a).reshape(a.shape[0], -1)
+    b = np.asarray(b).reshape(b.shape[0], -1)
+    return np.array([np.linalg.norm(a[v] - b[v]) / max(np.linalg.norm(b[v]), 1e-300) for v in range(a.shape[0])])
+
+
+def report(tag, e):
+    print(f"[configs] {tag}: " + " ".join(f"{x:.2e}" for x in e), flush=True)
+
+
+def run_both(case, nsteps, chunks=1):
+    g = case.load(capi.Solver(case.cfg))
+    o = case.load(O.Oracle(case.cfg))
+    for _ in range(chunks):
+        g.step(nsteps // chunks)
+        o.step(nsteps // chunks)
+        e = relerr(g.get_state(), o.get_state())
+        report(f"{case.name} after {g.get_flow()[1]} steps", e)
+        assert (e <= TOL).all(), e
+    return g, o
+
+
+def flags_equal(g, o):
+    for a, b in zip(g.get_flags(), o.get_flags()):
+        assert np.array_equal(a, b)
+
+
+# --------------------------------------------------------------------- C1
+def test_c1_tgv64_200_steps():
+    """The whole C1 run: ocfd_step replays a captured graph of two steps here
+    (no filter, no time-dependent boundary), so this also holds the graph path
+    to the oracle over 200 steps."""
+    case = cases.tgv(64, nsteps=200)
+    g, o = run_both(case, 200, chunks=2)
+    (pg, ig, tg), (po, io, to) = g.get_flow(), o.get_flow()
+    assert ig == io == 200 and tg == to
+
+
+# --------------------------------------------------------------------- C2
+def test_c2_ramp_full_size_20_steps():
+    case = cases.ramp(512, 128, 64, nsteps=20)
+    g, o = run_both(case, 20, chunks=2)
+    flags_equal(g, o)
+    pg, po = g.get_pressure(), o.get_pressure()
+    assert np.abs(pg - po).max() / np.abs(po).max() <= TOL
+
+
+# --------------------------------------------------------------------- C3
+def test_c3_shock_turbulence_10_steps_fp64_and_mixed():
+    case = cases.shock_turbulence(64, 32, 32, nsteps=10)
+    g, o = run_both(case, 10)
+    cfg32 = capi.OcfdConfig.from_buffer_copy(case.cfg)
+    cfg32.precision = capi.PREC_MIXED
+    m = case.load(capi.Solver(cfg32))
+    m.step(10)
+    l2 = rel_l2(m.get_state(), o.get_state())
+    mx = relerr(m.get_state(), o.get_state())
+    report("C3 64x32x32 mixed vs FP64 oracle, L2", l2)
+    report("C3 64x32x32 mixed vs FP64 oracle, max", mx)
+    assert (l2 <= 1e-6).all() and (mx <= 1e-4).all()
+
+
+def test_c3_full_size_rhs():
+    case = cases.shock_turbulence(512, 256, 256, nsteps=3)
+    g = case.load(capi.Solver(case.cfg))
+    g.rhs(1)
+    du = g.get_du()
+    print("[configs] C3 full size: oracle", flush=True)
+    o = case.load(O.Oracle(case.cfg))
+    o.rhs(1)
+    e = relerr(du, o.get_du())
+    report("C3 512x256x256 rhs", e)
+    assert (e <= TOL).all(), e
+    o.close()
+    del o, du
+    gc.collect()
+    g.step(3)
+    f = g.get_state()
+    assert np.isfinite(f).all()
+    assert f[0].min() > 0
+
+
+# --------------------------------------------------------------------- C4
+def bf16(a):
+    b = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32)
+
+
+def mlp_prob(p, F):
+    """Forward with the kernel's BF16 roundings (inputs, W1, W2, first hidden
+    activations)."""
+    a1 = bf16(M.gelu(bf16(F) @ bf16(p["W1"]).T + p["b1"]).astype(np.float32))
+    a2 = M.gelu(a1 @ bf16(p["W2"]).T + p["b2"])
+    return 1.0 / (1.0 + np.exp(-(a2 @ p["W3"] + p["b3"][0])))
+
+
+def shipped_params():
+    from pathlib import Path
+    import pyocfd
+    return np.load(Path(pyocfd.__file__).parent / "sensor_params.npy")
+
+
+def test_c4_learned_inference_on_a_ramp():
+    """The shipped network on a 64x40x12 ramp (IBC 108 ghosts, closures,
+    periodic z): probabilities against the NumPy forward on features from
+    the oracle's LAP fields (ghost layers as the reference leaves them), and
+    the flags by the flag rule."""
+    case = cases.ramp(64, 40, 12, ma=2.9, angle=24.0, nsteps=6, filt=False)
+    params = shipped_params()
+    g = case.load(capi.Solver(case.cfg))
+    o = case.load(O.Oracle(case.cfg))
+    g.step(3)
+    o.step(3)
+    assert (relerr(g.get_state(), o.get_state()) <= TOL).all()
+    g.set_sensor(params, 0.2, 0.8)
+    g.rhs(1)
+    prob = g.get_sensor_prob().reshape(-1)
+    d, u, v, w, T = [o.get_prim_lap(k) for k in range(5)]
+    A = np.stack([o.get_metric_lap(3 + m) for m in range(9)])
+    p00 = 1.0 / (case.cfg.gamma * case.cfg.ama ** 2)
+    nz, ny, nx = 12, 40, 64
+    nodes = [(i, j, k) for k in range(nz) for j in range(ny) for i in range(nx)]
+    F = M.features_field(d, u, v, w, T, A, p00, 1.0 / case.cfg.ama, nodes).astype(np.float32)
+    ref = mlp_prob(capi.mlp_unpack(params), F)
+    err = np.abs(prob - ref)
+    print(f"[configs] C4 ramp inference |prob - ref| max {err.max():.2e} mean {err.mean():.2e}", flush=True)
+    assert err.max() <= 5e-4 and err.mean() <= 2e-5
+    P = p00 * d * T
+    kp = M.jameson_kp(P[3:-3, 3:-3, 3:-3], *case.cfg.hy_dp_intv).reshape(-1)
+    want = M.flag_rule(prob, kp, 0.2, 0.8).reshape(nz, ny, nx)
+    fx, fy, fz = g.get_flags()
+    assert np.array_equal(fx[:, :, 1:], want)
+    assert np.array_equal(fy[:, 1:, :], want)
+    assert np.array_equal(fz[1:, :, :], want)
+
+
+def test_c4_full_size_learned_inference():
+    """1024x256x128 (C4's grid) with the shipped weights: two steps, then
+    probabilities on a seeded sample of interior nodes (at least 4 nodes
+    from every face: the sensor reads the ghost layers before this stage's
+    periodic fill, so they hold the previous stage's values) against the NumPy forward on
+    features from the GPU's own primitives, and the learned flags against
+    the plain sensor's."""
+    case = cases.ramp(1024, 256, 128, nsteps=4)
+    params = shipped_params()
+    g = case.load(capi.Solver(case.cfg))
+    g.step(2)
+    g.rhs(1)
+    plain = [f.copy() for f in g.get_flags()]
+    g.set_sensor(params, 0.2, 0.8)
+    g.rhs(1)
+    prob = g.get_sensor_prob()
+    learned = g.get_flags()
+    assert np.isfinite(prob).all() and prob.min() >= 0 and prob.max() <= 1
+    agree = np.mean([np.mean(a[1:-1, 1:-1, 1:-1] == b[1:-1, 1:-1, 1:-1]) for a, b in zip(learned, plain)])
+    print(f"[configs] C4 full size: learned flags equal to the plain ones at {agree:.5f} of the faces",
+          flush=True)
+    assert agree >= 0.99
+    (prims, _, _) = g.get_flow()
+    met = g.get_metrics()
+    def pad(a):
+        """LAP box, ghost layers edge-filled (never read by the sampled nodes)."""
+        return np.pad(a, 4, mode="edge")
+
+    d, u, v, w, T = [pad(a) for a in prims]
+    A = np.stack([pad(met[3 + m]) for m in range(9)])
+    rng = np.random.default_rng(4)
+    nodes = list(zip(rng.integers(4, 1024 - 4, 4000), rng.integers(4, 256 - 4, 4000), rng.integers(4, 128 - 4, 4000)))
+    p00 = 1.0 / (case.cfg.gamma * case.cfg.ama ** 2)
+    F = M.features_field(d, u, v, w, T, A, p00, 1.0 / case.cfg.ama, nodes).astype(np.float32)
+    ref = mlp_prob(capi.mlp_unpack(params), F)
+    got = np.array([prob[k, j, i] for (i, j, k) in nodes])
+    err = np.abs(got - ref)
+    print(f"[configs] C4 full size |prob - ref| on 4000 nodes: max {err.max():.2e} mean {err.mean():.2e}",
+          flush=True)
+    assert err.max() <= 5e-4 and err.mean() <= 2e-5
+
+
+# --------------------------------------------------------------------- C5
+def test_c5_swept_ramp_10_steps():
+    case = cases.ramp(64, 40, 16, sweep=30.0, nsteps=10, filter_every=5)
+    g, o = run_both(case, 10, chunks=2)
+    flags_equal(g, o)
+
+
+def test_c5_full_size_rhs_and_steps():
+    case = cases.ramp(1024, 384, 256, sweep=30.0, nsteps=4)
+    g = case.load(capi.Solver(case.cfg))
+    g.rhs(1)
+    du = g.get_du()
+    flags = [f.copy() for f in g.get_flags()]
+    print("[configs] C5 full size: oracle", flush=True)
+    o = case.load(O.Oracle(case.cfg))
+    o.rhs(1)
+    e = relerr(du, o.get_du())
+    report("C5 1024x384x256 rhs", e)
+    for a, b in zip(flags, o.get_flags()):
+        assert np.array_equal(a, b)
+    assert (e <= TOL).all(), e
+    o.close()
+    del o, du
+    gc.collect()
+    g.step(4)
+    (p, istep, _) = g.get_flow()
+    assert istep == 4
+    assert all(np.isfinite(a).all() for a in p)
+    assert np.all(p[4][:, 0, :] == case.cfg.bc_rpara[0])      # isothermal wall
+    assert not p[1][:, 0, :].any()                            # no slip
+    for fl in g.get_flags():
+        inner = fl[1:-1, 1:-1, 1:-1]
+        assert inner.min() >= 1 and inner.max() <= 3
