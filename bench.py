@@ -1,19 +1,23 @@
 #!/usr/bin/env python
 """Throughput of the OpenCFD-SCU right-hand-side path on one MI355X.
 
-Workload (N=1): the 512x128x64 compression ramp - hybrid_auto WENO7/UP7
-inviscid fluxes on the curvilinear (tanh-stretched) grid, CD8 viscous terms,
-IBC 108 inflow/isothermal wall, non-reflecting outflow/top, the
-shock-capturing filter every 10 steps, SSP-RK3 - plain FP64.  Synthetic
-seeded inputs.  A step is one RK3 time step (three right-hand sides and
-stages); the metric is grid-point RK-stage updates per second.
+Workload (N=1): the 1024x384x256 swept compression ramp (24-degree ramp
+swept 30 degrees in z, 100.7 M points) - hybrid_auto WENO7/UP7 inviscid
+fluxes on the curvilinear (tanh-stretched) grid, CD8 viscous terms, IBC 108
+inflow/isothermal wall, non-reflecting outflow/top, the shock-capturing
+filter every 10 steps, SSP-RK3 - plain FP64.  Synthetic seeded inputs.  A
+step is one RK3 time step (three right-hand sides and stages); the metric is
+grid-point RK-stage updates per second.  --nx/--ny/--nz/--sweep select
+another ramp (e.g. 512x128x64 unswept).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  `roofline` covers the kernel with the largest share
 of the timed region, timed with HIP events on the library's stream inside the
 timed region; `cpu_baseline` times the CPU oracle (the C restatement of
-the same routines, OpenMP over the point loops) on one step of the same grid.
+the same routines, OpenMP over the point loops) on --cpu-sample-steps full
+steps of the same grid (default 1 on grids above 50 M points, where a step
+takes about 17 s on 16 threads, else 15).
 """
 import argparse
 import json
@@ -48,15 +52,16 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--ny", type=int, default=128)
-    ap.add_argument("--nz", type=int, default=64)
-    ap.add_argument("--sweep", type=float, default=0.0, help="ramp sweep angle in degrees (30 for the swept ramp)")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=384)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--sweep", type=float, default=30.0, help="ramp sweep angle in degrees (0 for the unswept ramp)")
     ap.add_argument("--sensor-params", default=str(REPO / "opencfd-scu_amd" / "pyocfd" / "sensor_params.npy"),
                     help="learned-sensor weights for the mixed + learned-sensor leg (skipped if absent)")
-    ap.add_argument("--cpu-sample-steps", type=int, default=15)
+    ap.add_argument("--cpu-sample-steps", type=int, default=0,
+                    help="full steps of the CPU oracle (0: 1 on grids above 50 M points, else 15)")
     ap.add_argument("--profile-steps", type=int, default=10, help="steps of the per-kernel timing pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
@@ -219,7 +224,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and int(os.environ.get("RANK", "0")) == 0:
         log("timing the CPU oracle")
-        cpu = cpu_baseline(case, args.cpu_sample_steps)
+        cpu_steps = args.cpu_sample_steps or (1 if npts > 50_000_000 else 15)
+        cpu = cpu_baseline(case, cpu_steps)
     out = {
         "metric": "grid-point RK-stage updates/sec on 1 MI355X",
         "value": value,

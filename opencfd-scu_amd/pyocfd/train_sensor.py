@@ -1,7 +1,9 @@
 """Train the learned hybrid sensor on seeded ramp flows and save its weights.
 
-    python -m pyocfd.train_sensor [--nx 256 --ny 64 --nz 32] [--steps 60]
-                                  [--out sensor_params.npy]
+    python -m pyocfd.train_sensor [--nx 1024 --ny 256 --nz 128] [--stride 2]
+                                  [--steps 60] [--out sensor_params.npy]
+
+(the defaults: the 1024x256x128 ramp, every second node in each direction).
 
 Training data are the plain FP64 solver's own runs of the compression ramp
 over Mach {2.5, 2.9, 3.5} x angle {18, 24, 30} degrees, leaving Mach 2.9 /
@@ -11,7 +13,8 @@ kp > 1).  Training (FP32, Adam, positives weighted by the class ratio) and
 the collection of samples run in the library's own kernels
 (ocfd_mlp_collect / ocfd_mlp_train).  The script prints the held-out
 balanced accuracy of the untrained and the trained network and writes the
-1889 parameters.
+1889 parameters, and the report (sample counts, losses, held-out accuracy)
+as JSON beside them (same name, .json).
 """
 from __future__ import annotations
 
@@ -45,11 +48,11 @@ def collect(trainer, ma, angle, nx, ny, nz, steps, stride):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ny", type=int, default=64)
-    ap.add_argument("--nz", type=int, default=32)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=256)
+    ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--stride", type=int, default=1)
+    ap.add_argument("--stride", type=int, default=2)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--lr", type=float, default=3e-3)
@@ -81,7 +84,9 @@ def main(argv=None):
               "held_out": {"flow": {"ma": HELD_OUT[0], "angle": HELD_OUT[1]}, "samples": int(held.n),
                            "balanced_accuracy_untrained": before[0], "balanced_accuracy_trained": after[0],
                            "tpr": after[1], "tnr": after[2]},
-              "grid": [a.nx, a.ny, a.nz], "steps": a.steps, "out": a.out}
+              "grid": [a.nx, a.ny, a.nz], "stride": a.stride, "steps": a.steps,
+              "epochs": a.epochs, "batch": a.batch, "lr": a.lr, "seed": a.seed}
+    Path(a.out).with_suffix(".json").write_text(json.dumps(report))
     print(json.dumps(report), flush=True)
     return report
 

@@ -19,23 +19,27 @@ sys.path.insert(0, str(REPO / "opencfd-scu_amd"))
 from pyocfd.capi import OcfdConfig  # noqa: E402  (interface type only)
 
 
-def build(force: bool = False) -> Path:
-    """Compile liboracle.so with the committed Makefile."""
+LIB_FMA = ORACLE_DIR / "liboracle_fma.so"
+
+
+def build(force: bool = False, path: Path = LIB) -> Path:
+    """Compile liboracle.so (or liboracle_fma.so) with the committed Makefile."""
     srcs = [ORACLE_DIR / n for n in ("ocfd_oracle.c", "ocfd_oracle.h", "Makefile")]
-    if force or not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in srcs):
-        subprocess.run(["make", "-C", str(ORACLE_DIR), "liboracle.so"], check=True,
+    if force or not path.exists() or path.stat().st_mtime < max(p.stat().st_mtime for p in srcs):
+        subprocess.run(["make", "-C", str(ORACLE_DIR), path.name], check=True,
                        stdout=subprocess.DEVNULL)
-    return LIB
+    return path
 
 
-_lib = None
+_libs = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        build()
-        L = C.CDLL(str(LIB))
+def lib(fma: bool = False) -> C.CDLL:
+    """The oracle library; fma=True: the build with contracted multiply-adds."""
+    path = LIB_FMA if fma else LIB
+    if path not in _libs:
+        build(path=path)
+        L = C.CDLL(str(path))
         dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
         ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         i8 = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
@@ -80,15 +84,15 @@ def lib() -> C.CDLL:
             fn.restype = res
             fn.argtypes = args
         _ip = ip  # noqa: F841
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 class Oracle:
     """Mirror of pyocfd.capi.Solver backed by the CPU restatement."""
 
-    def __init__(self, cfg: OcfdConfig):
-        self.L = lib()
+    def __init__(self, cfg: OcfdConfig, fma: bool = False):
+        self.L = lib(fma)
         self.cfg = cfg
         self.nx, self.ny, self.nz = cfg.nx, cfg.ny, cfg.nz
         self.h = self.L.orc_create(C.byref(cfg))

@@ -9,7 +9,13 @@ through Opencfd.msg.
   it also needs come from cases.deck_ramp (the reference's own generators are
   separate MPI programs).  Bar: the OCFD00001000.dat the driver writes equals
   the oracle's primitives after 1000 steps to a relative max-norm of 1e-12
-  per variable.
+  per variable, or, where rounding alone moves this long run further, to
+  three times what it moves the oracle itself: the same oracle built with
+  contracted multiply-adds (oracle/liboracle_fma.so, which is what the GPU
+  compiler does, in other places) differs from the plain build by a few
+  1e-12 after 1000 steps of this deck (WENO7-SYMBO's nonlinear weights and
+  TV limiter switch amplify last-bit differences), and the GPU differs from
+  the oracle by the same amount.
 * Restart (src/OCFD_IO.c:37-91 reads Opencfd.msg as one int32, then
   OCFD%08d.dat; src/OCFD_NS_Solver.cu:63-167 runs while tt < END_TIME): a run
   restarted from the flow file of step k reaches the same state at step N as
@@ -103,20 +109,39 @@ def test_reference_deck_1000_steps_through_the_driver(tmp_path):
     o.step(1000)
     po, io, to = o.get_flow()
     assert io == 1000 and tt == to
-    e = relerr(prims, po)
-    print("[deck] flow file after 1000 steps vs oracle: " + " ".join(f"{x:.2e}" for x in e), flush=True)
-    assert all(np.isfinite(a).all() for a in prims)
+    print("[deck] oracle with contracted multiply-adds, 1000 steps", flush=True)
+    f = case.load(O.Oracle(case.cfg, fma=True))
+    f.step(1000)
+    pf = f.get_flow()[0]
     # w is zero at the start and stays at the round-off of the blowing (z
     # periodic, nothing drives it): measured against the scale of u
-    assert max(e[:3] + e[4:]) <= 1e-12, e
-    assert np.abs(prims[3] - po[3]).max() <= 1e-12 * np.abs(po[1]).max()
+    def dev(a, b):
+        e = relerr(a, b)
+        e[3] = float(np.abs(a[3] - b[3]).max() / np.abs(b[1]).max())
+        return e
+
+    e, noise = dev(prims, po), dev(pf, po)
+    print("[deck] flow file after 1000 steps vs oracle: " + " ".join(f"{x:.2e}" for x in e), flush=True)
+    print("[deck] oracle (fma) vs oracle:               " + " ".join(f"{x:.2e}" for x in noise), flush=True)
+    assert all(np.isfinite(a).all() for a in prims)
+    for v in range(5):
+        assert e[v] <= max(1e-12, 3.0 * noise[v]), (v, e, noise)
 
 
 @pytest.mark.gpu
 def test_restart_through_opencfd_msg(tmp_path):
     """Continuous: 6 steps with a flow file every 3.  Restarted: Opencfd.msg
     names step 3, the run reads OCFD00000003.dat and goes on to END_TIME.
-    The filter runs every 2 steps, so step 3's primitives carry no lag of it."""
+    The filter runs every 2 steps, so step 3's primitives carry no lag of it.
+
+    The restart rebuilds the conservative state from the saved primitives
+    (a prim -> cons round trip), so it differs from the continuous run by
+    what three more steps make of last-bit differences.  Bars: the
+    restarted GPU run equals the oracle restarted from the same flow file
+    to 1e-12 per variable (the same inputs and round trip on both sides);
+    it differs from the continuous GPU run by at most three times what the
+    restart moves the oracle's own run (or 1e-12).  w, which only the
+    seeded wall blowing drives, is measured against the scale of u."""
     n, k = 6, 3
     case = cases.ramp(48, 32, 10, nsteps=n, filter_every=2)
     case.cfg.kstep_save = k
@@ -132,11 +157,28 @@ def test_restart_through_opencfd_msg(tmp_path):
     pa, ia, ta = capi.read_flow_file(a / f"OCFD{n:08d}.dat", 48, 32, 10)
     pb, ib, tb = capi.read_flow_file(b / f"OCFD{n:08d}.dat", 48, 32, 10)
     assert ia == ib == n and ta == tb
-    e = relerr(pb, pa)
-    print("[deck] restarted vs continuous: " + " ".join(f"{x:.2e}" for x in e), flush=True)
-    # the restart rebuilds the conservative state from the saved primitives
-    # (a prim -> cons round trip), so the two agree to round-off, not bits
-    assert max(e) <= 1e-12, e
+
+    def dev(x, y):
+        e = relerr(x, y)
+        e[3] = float(np.abs(x[3] - y[3]).max() / np.abs(y[1]).max())
+        return e
+
+    # the oracle, continuous and restarted from the same flow file
     o = case.load(O.Oracle(case.cfg))
     o.step(n)
-    assert max(relerr(pb, o.get_flow()[0])) <= 1e-12
+    po = o.get_flow()[0]
+    pk, ik, tk = capi.read_flow_file(a / f"OCFD{k:08d}.dat", 48, 32, 10)
+    r = O.Oracle(case.cfg)
+    r.load_mesh(*case.mesh)
+    r.load_flow(*pk, istep=ik, tt=tk)
+    r.set_inlet_profile(case.inlet)
+    r.init()
+    r.step(n - k)
+    pr = r.get_flow()[0]
+    e_gpu, e_ref, noise = dev(pb, pa), dev(pb, pr), dev(pr, po)
+    print("[deck] restarted vs continuous (GPU):       " + " ".join(f"{x:.2e}" for x in e_gpu), flush=True)
+    print("[deck] restarted GPU vs restarted oracle:   " + " ".join(f"{x:.2e}" for x in e_ref), flush=True)
+    print("[deck] restarted vs continuous (oracle):    " + " ".join(f"{x:.2e}" for x in noise), flush=True)
+    assert max(e_ref) <= 1e-12, e_ref
+    for v in range(5):
+        assert e_gpu[v] <= max(1e-12, 3.0 * noise[v]), (v, e_gpu, noise)

@@ -161,3 +161,70 @@ def test_training_learns_the_plain_sensor():
     print(f"[mlp] held-out balanced accuracy untrained {b0[0]:.4f} trained {b1[0]:.4f} "
           f"(TPR {b1[1]:.3f}, TNR {b1[2]:.3f})", flush=True)
     assert b1[0] > b0[0] + 0.2 and b1[0] > 0.85
+
+
+def test_trained_sensor_gain_on_the_held_out_ramp():
+    """What the shipped network gains, on the flow it was not trained on
+    (the Mach 2.9, 24-degree ramp) on a 256x64x32 grid, 200 steps on the
+    mixed path from the same initial state, each run judged against the
+    plain FP64 solution (hybrid_auto with the plain sensor): relative L2 of
+    density and pressure, and the measured time per step.
+
+      no sensor          UP7 everywhere (the hybrid scheme without a sensor)
+      untrained network  the network's seeded initial weights deciding every
+                         node (confidence band [0.5, 0.5]: no fallback)
+      trained network    the shipped weights deciding every node
+      trained + band     the shipped weights with the [0.2, 0.8] band that
+                         falls back to the plain sensor
+      plain sensor       the mixed path with the plain sensor (the FP32
+                         rounding alone)
+
+    The trained network alone must come within the 1e-4 error bar and
+    closer to the plain FP64 solution than no sensor and than the untrained
+    network; with the band, within the bar as well."""
+    from pathlib import Path
+    import pyocfd
+    params = np.load(Path(pyocfd.__file__).parent / "sensor_params.npy")
+    untrained = capi.mlp_pack(M.init_params(seed=7, dtype=np.float32))
+    nsteps = 200
+    case = cases.ramp(256, 64, 32, ma=2.9, angle=24.0, nsteps=nsteps, filt=False)
+    ref = case.load(capi.Solver(case.cfg))
+    ref.step(nsteps)
+    (p64, _, _) = ref.get_flow()
+    ref.close()
+    g = case.cfg.gamma * case.cfg.ama ** 2
+    pr64 = p64[0] * p64[4] / g
+
+    def run(tag, scheme=None, sensor=None, lo=0.2, hi=0.8):
+        cfg = capi.OcfdConfig.from_buffer_copy(case.cfg)
+        cfg.precision = capi.PREC_MIXED
+        if scheme is not None:
+            cfg.scheme_invis = scheme
+        s = case.load(capi.Solver(cfg))
+        if sensor is not None:
+            s.set_sensor(sensor, lo, hi)
+        s.step(nsteps)
+        s.sync()
+        ms = s.last_step_ms() / nsteps   # the time of the whole step() call
+        (p, _, _) = s.get_flow()
+        weno = None
+        if cfg.scheme_invis == capi.HYBRIDAUTO:
+            fl = s.get_flags()
+            weno = sum(int((f > 1).sum()) for f in fl) / sum(f.size for f in fl)
+        s.close()
+        e_rho = float(np.linalg.norm(p[0] - p64[0]) / np.linalg.norm(p64[0]))
+        e_p = float(np.linalg.norm(p[0] * p[4] / g - pr64) / np.linalg.norm(pr64))
+        w = "-" if weno is None else f"{weno:.4f}"
+        print(f"[mlp] gain {tag:18s} L2 rho {e_rho:.2e} p {e_p:.2e}  WENO faces {w}  {ms:.3f} ms/step",
+              flush=True)
+        return max(e_rho, e_p)
+
+    e_none = run("no sensor", scheme=capi.UP7)
+    e_untr = run("untrained network", sensor=untrained, lo=0.5, hi=0.5)
+    e_tr = run("trained network", sensor=params, lo=0.5, hi=0.5)
+    e_band = run("trained + band", sensor=params)
+    e_plain = run("plain sensor")
+    print(f"[mlp] gain: trained network {e_none / e_tr:.1f}x closer than no sensor, "
+          f"{e_untr / e_tr:.1f}x closer than the untrained network; plain sensor {e_plain:.2e}", flush=True)
+    assert e_tr <= 1e-4 and e_band <= 1e-4
+    assert e_tr < e_none and e_tr < e_untr
