@@ -195,6 +195,21 @@ int ocfd_get_metrics(ocfd_ctx *ctx, double *arrays13);
  * field including its ghost layers.  Any pointer may be null. */
 int ocfd_diagnostics(ocfd_ctx *ctx, double *mean_energy, double *mean_T,
                      long long *n_nan_T, long long *n_neg_T);
+/* ocfd_diagnostics without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned host
+ * memory (code under HIP graph capture elsewhere in the process, serving
+ * loops, harnesses that forbid hipMalloc).  The partial sums and counts live
+ * in one file-scope device array of OCFD_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends; its address is fetched by ocfd_create);
+ * the results return through a 32-byte copy.  Grids whose partials do not
+ * fit, (ceil(nx/64) ceil(ny/4) nz) > 523,774 (e.g. above 1024x384x256 by a
+ * third), are refused with OCFD_ERR_UNSUPPORTED.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two contexts on
+ * different host threads) race.  Same values as ocfd_diagnostics (the counts
+ * exactly, the means up to the order of the final sum). */
+#define OCFD_NOALLOC_SCRATCH_BYTES (8LL << 20)
+int ocfd_diagnostics_noalloc(ocfd_ctx *ctx, double *mean_energy, double *mean_T,
+                             long long *n_nan_T, long long *n_neg_T);
 /* Write OCFD3d-Jacobi.dat (the 13 arrays above) so that a later run reads the
  * metrics instead of deriving them from OCFD3d-Mesh.dat.  The reference only
  * reads this file (src/OCFD_Comput_Jacobian3d.cu:148-215) and has no writer. */

@@ -666,6 +666,47 @@ __global__ void __launch_bounds__(256) k_diag_scan(const S* __restrict__ T, long
     }
 }
 
+// Final reduction of the diagnostics partials on the device (one block):
+// thread t sums partials t, t+256, ... in order, then an LDS tree; out =
+// {sum E, sum T, NaN count, negative count} (the counts as long long bits).
+__global__ void __launch_bounds__(256) k_diag_final(const double* __restrict__ partE,
+                                                    const double* __restrict__ partT, long long nb,
+                                                    const int* __restrict__ scan_part, double* out) {
+    __shared__ double se[256], st[256];
+    __shared__ long long sn[256], sg[256];
+    const int tid = threadIdx.x;
+    double e = 0.0, t = 0.0;
+    for (long long b = tid; b < nb; b += 256) {
+        e += partE[b];
+        t += partT[b];
+    }
+    long long nn = 0, ng = 0;
+    for (int b = tid; b < DIAG_SCAN_BLOCKS; b += 256) {
+        nn += scan_part[2 * b];
+        ng += scan_part[2 * b + 1];
+    }
+    se[tid] = e;
+    st[tid] = t;
+    sn[tid] = nn;
+    sg[tid] = ng;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            se[tid] += se[tid + w];
+            st[tid] += st[tid + w];
+            sn[tid] += sn[tid + w];
+            sg[tid] += sg[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = se[0];
+        out[1] = st[0];
+        reinterpret_cast<long long*>(out)[2] = sn[0];
+        reinterpret_cast<long long*>(out)[3] = sg[0];
+    }
+}
+
 template <typename To, typename From>
 __global__ void k_convert(To* dst, const From* src, long long n) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -792,6 +833,11 @@ void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, 
     hipLaunchKernelGGL(k_diag_sums<S>, pgrid(g), dim3(64, 4), 0, st, g, E, T_base, partE, partT);
     // the LAP box is the first sz*(nz+8) elements of the field
     hipLaunchKernelGGL(k_diag_scan<S>, dim3(DIAG_SCAN_BLOCKS), dim3(256), 0, st, T_base, g.sz * (g.nz + 8), scan_part);
+}
+
+void launch_diag_final(const double* partE, const double* partT, long long nb, const int* scan_part, double* out4,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, st, partE, partT, nb, scan_part, out4);
 }
 
 #define OCFD_INSTANTIATE(S)                                                                                     \

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -830,6 +831,39 @@ int init_fields(ocfd_ctx* x) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of ocfd_diagnostics_noalloc: one file-scope device array, which
+// every GPU the library runs on holds at its own address until the process
+// ends.  In host code `scratch` is only a stand-in for the symbol: its device
+// address is fetched once per device (noalloc_setup, called by ocfd_create
+// with the device current) and only that address is used afterwards.
+// Layout in doubles: [0,4) results {sum E, sum T, NaN count, negative count},
+// then the partial sums of E and of T (diag_sum_blocks each), then the
+// 2*DIAG_SCAN_BLOCKS int counts.
+// ---------------------------------------------------------------------------
+constexpr long long NOALLOC_DOUBLES = OCFD_NOALLOC_SCRATCH_BYTES / 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(OCFD_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return 0;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(OCFD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<double*>(p);
+    return 0;
+}
+
+long long noalloc_doubles_needed(const Geo& g) {
+    return 4 + 2LL * diag_sum_blocks(g) + DIAG_SCAN_BLOCKS;  // the counts: 2 ints per scan block
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -879,6 +913,10 @@ int ocfd_create(const ocfd_config* cfg, int device, ocfd_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) {
         delete x;
         return fail(OCFD_ERR_HIP, "hipSetDevice failed");
+    }
+    if (const int rc = noalloc_setup(device)) {
+        delete x;
+        return rc;
     }
     if (hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&x->st2, hipStreamNonBlocking) != hipSuccess ||
@@ -1309,6 +1347,43 @@ int ocfd_diagnostics(ocfd_ctx* x, double* mean_energy, double* mean_T, long long
     if (mean_T) *mean_T = st / npts;
     if (n_nan_T) *n_nan_T = nn;
     if (n_neg_T) *n_neg_T = ng;
+    return 0;
+}
+
+int ocfd_diagnostics_noalloc(ocfd_ctx* x, double* mean_energy, double* mean_T, long long* n_nan_T,
+                             long long* n_neg_T) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    const Geo& g = x->g;
+    const long long need = noalloc_doubles_needed(g);
+    if (need > NOALLOC_DOUBLES)
+        return fail(OCFD_ERR_UNSUPPORTED,
+                    "ocfd_diagnostics_noalloc: this grid needs " + std::to_string(need * 8) +
+                        " bytes of scratch, above the " + std::to_string(NOALLOC_DOUBLES * 8) +
+                        " bytes of OCFD_NOALLOC_SCRATCH_BYTES (use ocfd_diagnostics)");
+    if (!x->inited) return fail(OCFD_ERR_STATE, "context not initialised");
+    double* s = g_scratch_dev[x->device];
+    if (!s) return fail(OCFD_ERR_STATE, "scratch address not set up for this device");
+    const long long nb = diag_sum_blocks(g);
+    double* partE = s + 4;
+    double* partT = partE + nb;
+    int* scan = reinterpret_cast<int*>(partT + nb);
+    const double* E = x->f + 4 * x->fs;
+    // every partial, count and result slot is written by these launches:
+    // nothing left by an earlier call is read
+    if (x->mixed) launch_diag(g, E, x->sT, partE, partT, scan, x->st);
+    else launch_diag(g, E, x->T, partE, partT, scan, x->st);
+    launch_diag_final(partE, partT, nb, scan, s, x->st);
+    HIPCK(hipGetLastError());
+    double out[4];
+    HIPCK(hipMemcpyAsync(out, s, sizeof(out), hipMemcpyDeviceToHost, x->st));
+    HIPCK(hipStreamSynchronize(x->st));
+    long long cnt[2];
+    std::memcpy(cnt, out + 2, sizeof(cnt));
+    const double npts = (double)g.nx * g.ny * g.nz;
+    if (mean_energy) *mean_energy = out[0] / npts;
+    if (mean_T) *mean_T = out[1] / npts;
+    if (n_nan_T) *n_nan_T = cnt[0];
+    if (n_neg_T) *n_neg_T = cnt[1];
     return 0;
 }
 

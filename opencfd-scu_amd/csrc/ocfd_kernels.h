@@ -191,6 +191,10 @@ int diag_sum_blocks(const Geo& g);
 template <typename S>
 void launch_diag(const Geo& g, const double* E, const S* T_base, double* partE, double* partT, int* scan_part,
                  hipStream_t st);
+// The partials above reduced on the device by one block: out4 = {sum E,
+// sum T, NaN count, negative count} (the counts stored as long long).
+void launch_diag_final(const double* partE, const double* partT, long long nb, const int* scan_part, double* out4,
+                       hipStream_t st);
 // Running time average of the primitives: avg (5 FP64 fields, stride fs)
 // becomes (n avg + x)/(n + 1) at every interior point.
 template <typename S>

@@ -201,6 +201,8 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "ocfd_set_sensor_from": (C.c_int, [vp, vp, C.c_float, C.c_float]),
         "ocfd_diagnostics": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                        C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "ocfd_diagnostics_noalloc": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "ocfd_get_hybrid_flags": (C.c_int, [vp, i8p, i8p, i8p]),
         "ocfd_analysis": (C.c_int, [vp, C.c_int, C.c_int]),
         "ocfd_get_q": (C.c_int, [vp, dp]),
@@ -446,6 +448,14 @@ class Solver:
         """(mean total energy, mean T, NaN count of T, negative-T count)."""
         e, t, nn, ng = C.c_double(), C.c_double(), C.c_longlong(), C.c_longlong()
         _check(self.lib.ocfd_diagnostics(self.h, C.byref(e), C.byref(t), C.byref(nn), C.byref(ng)), "diagnostics")
+        return e.value, t.value, nn.value, ng.value
+
+    def diagnostics_noalloc(self):
+        """The same through ocfd_diagnostics_noalloc (no allocation: a
+        file-scope device scratch array)."""
+        e, t, nn, ng = C.c_double(), C.c_double(), C.c_longlong(), C.c_longlong()
+        _check(self.lib.ocfd_diagnostics_noalloc(self.h, C.byref(e), C.byref(t), C.byref(nn), C.byref(ng)),
+               "diagnostics_noalloc")
         return e.value, t.value, nn.value, ng.value
 
     def analysis(self, style, event=0):
