@@ -257,6 +257,77 @@ __device__ __forceinline__ void face_all(LD&& ld, int c, int r, int bl, int br, 
     face_all_m<R, W, SCH>(ld, c, r, bl, br, hy, hm);
 }
 
+// The closure/scheme case of a face's P value (0..3 left closures by c, 4..6
+// right closures, 7 the scheme, 8 WENO7, 9 WENO5) and M value (0..2 left
+// closures, 3..6 right closures, 7 the scheme, 8 WENO7, 9 WENO5): the
+// selection of face_all_p / face_all_m.
+template <int SCH>
+__device__ __forceinline__ int case_p(int c, int r, int bl, int br, int hy) {
+    constexpr int kaP = Ext<SCH, 0>::ka, kbP = Ext<SCH, 0>::kb;
+    if (bl && c <= -kaP) return c;
+    if (br && r >= -kbP) return r == -1 ? 4 : r == -2 ? 5 : 6;
+    if (SCH == OCFD_HYBRIDAUTO) return hy == 1 ? 7 : hy == 2 ? 8 : 9;
+    return 7;
+}
+template <int SCH>
+__device__ __forceinline__ int case_m(int c, int r, int bl, int br, int hy) {
+    constexpr int kaM = Ext<SCH, 1>::ka, kbM = Ext<SCH, 1>::kb;
+    if (bl && c < -kaM) return c < 2 ? c : 2;
+    if (br && r >= -kbM - 1) return r == -1 ? 3 : r == -2 ? 4 : r == -3 ? 5 : 6;
+    if (SCH == OCFD_HYBRIDAUTO) return hy == 1 ? 7 : hy == 2 ? 8 : 9;
+    return 7;
+}
+// One P value (case pc) or M value (case mc) of hybrid_auto from the stencil
+// g[0..7] of nodes j-3..j+4: the formulas of face_all_p / face_all_m case by
+// case.
+template <typename R, typename W>
+__device__ __forceinline__ R recon_p_hybrid(int pc, const R* g) {
+    switch (pc) {
+        case 0: return g[4];
+        case 1: return weno5_up_left<R, W>(g + 1);
+        case 2: return weno5_up_leftplus<R, W>(g + 1);
+        case 4: return g[3] + R(0.5) * d_minmod2(g[3] - g[2], g[3] - g[2]);
+        case 5: return weno5_up_rightplus<R, W>(g + 1);
+        case 7: return up7_up(g);
+        case 8: return weno7_up<R, W>(g);
+        default: return weno5_up<R, W>(g + 1);  // 3, 6, 9
+    }
+}
+template <typename R, typename W>
+__device__ __forceinline__ R recon_m_hybrid(int mc, const R* g) {
+    switch (mc) {
+        case 0: return weno5_dn_left<R, W>(g + 2);
+        case 1: return weno5_dn_leftplus<R, W>(g + 2);
+        case 3: return g[3];
+        case 4: return g[4] - R(0.5) * d_minmod2(g[4] - g[3], g[4] - g[3]);
+        case 5: return weno5_dn_rightplus<R, W>(g + 2);
+        case 7: return up7_dn(g);
+        case 8: return weno7_dn<R, W>(g + 1);
+        default: return weno5_dn<R, W>(g + 2);  // 2, 6, 9
+    }
+}
+
+// hybrid_auto sends most faces to UP7 and a few (at shocks, and closures near
+// DIF_BOUND faces) to WENO7 / WENO5.  A wave whose lanes each ran their own
+// face's ten reconstructions would run the WENO code ten times over for a
+// single WENO face while the other lanes wait.  Instead the UP7 faces are
+// computed by their own lanes and the ten P and M values of the other
+// ("special") faces are spread over the wave, one value per lane: up to six
+// special faces per wave in one pass; a wave with more takes the per-face
+// path.  Same formulas, so the same values.  Used in the x sweep, which is
+// VALU-bound (91 % VALU busy on the 1024x384x256 swept ramp; 2.40 -> 2.23 ms
+// per launch); the y and z tiles, at 48-60 % VALU busy, measured the same or
+// 2 % slower with it and keep the per-face path.
+#ifndef OCFD_SWEEP_COMPACT
+#define OCFD_SWEEP_COMPACT 1
+#endif
+constexpr int COMPACT_MAX = 6;  // special faces per wave spread in one pass (10 values each)
+// The lane of the n-th special face of the wave (n-th set bit of mask).
+__device__ __forceinline__ int nth_lane(unsigned long long mask, int n) {
+    for (int t = 0; t < n; t++) mask &= mask - 1;
+    return mask ? __builtin_ctzll(mask) : 0;
+}
+
 // x sweep: a block owns XA rows of 56 output nodes (one wave per row); the
 // f+/f- of the 64 nodes of a row (4 ghost nodes each side) are staged in LDS.
 #ifndef OCFD_XA
@@ -322,17 +393,69 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
 
     // ---- phase 2: one half node per thread (face jf+1/2)
     S hp[5], hm[5];
-    if (fvalid) {
-        const int c = jf + 1, r = jf - n_dir;
-        face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br, hy,
-                                 S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
-    }
-    __syncthreads();
-    if (fthread) {
+    if (SCH == OCFD_HYBRIDAUTO && OCFD_SWEEP_COMPACT) {
+        int pc = 7, mc = 7;
+        if (fvalid) {
+            const int c = jf + 1, r = jf - n_dir;
+            pc = case_p<SCH>(c, r, A.bl, A.br, hy);
+            mc = case_m<SCH>(c, r, A.bl, A.br, hy);
+        }
+        const bool special = fvalid && (pc != 7 || mc != 7);
+        if (fvalid && !special) {
 #pragma unroll
-        for (int m = 0; m < 5; m++) {
-            sh[(m * XA + a_loc) * XF + f_loc] = fvalid ? hp[m] : S(0);
-            sh[((5 + m) * XA + a_loc) * XF + f_loc] = fvalid ? hm[m] : S(0);
+            for (int m = 0; m < 5; m++) {
+                S gp[8], gm[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    gp[k] = sh[at(m, f_loc + k, a_loc)];
+                    gm[k] = sh[at(5 + m, f_loc + k, a_loc)];
+                }
+                hp[m] = up7_up(gp);
+                hm[m] = up7_dn(gm);
+            }
+        }
+        const unsigned long long smask = __ballot(special);
+        const int nspec = __popcll(smask);
+        const bool spread = nspec <= COMPACT_MAX;
+        // job of this lane: value v (P of variable v < 5, M of variable v-5)
+        // of the special face held by lane fl
+        const bool job = spread && lane < 10 * nspec;
+        const int fl = nth_lane(smask, job ? lane / 10 : 0), v = lane % 10;
+        const int pcj = __shfl(pc, fl), mcj = __shfl(mc, fl);
+        S res = S(0);
+        if (job) {
+            S gg[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) gg[k] = sh[at(v, fl + k, a_loc)];
+            res = v < 5 ? recon_p_hybrid<S, double>(pcj, gg) : recon_m_hybrid<S, double>(mcj, gg);
+        }
+        if (special && !spread) {
+            const int c = jf + 1, r = jf - n_dir;
+            face_all<S, double, SCH>([&](int vv, int k) { return sh[at(vv, f_loc + k, a_loc)]; }, c, r, A.bl, A.br,
+                                     hy, S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
+        }
+        __syncthreads();
+        if (fthread && !(special && spread)) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                sh[(m * XA + a_loc) * XF + f_loc] = fvalid ? hp[m] : S(0);
+                sh[((5 + m) * XA + a_loc) * XF + f_loc] = fvalid ? hm[m] : S(0);
+            }
+        }
+        if (job) sh[(v * XA + a_loc) * XF + fl] = res;
+    } else {
+        if (fvalid) {
+            const int c = jf + 1, r = jf - n_dir;
+            face_all<S, double, SCH>([&](int v, int k) { return sh[at(v, f_loc + k, a_loc)]; }, c, r, A.bl, A.br,
+                                     hy, S(A.c.tv_lim), S(A.c.tv_max), hp, hm);
+        }
+        __syncthreads();
+        if (fthread) {
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                sh[(m * XA + a_loc) * XF + f_loc] = fvalid ? hp[m] : S(0);
+                sh[((5 + m) * XA + a_loc) * XF + f_loc] = fvalid ? hm[m] : S(0);
+            }
         }
     }
     __syncthreads();
