@@ -22,6 +22,21 @@
  * Hyscheme_flag only for variable 0 (src/OCFD_Schemes_hybrid_auto.cu:874,907,
  * 971,1005) and leave it uninitialised for variables 1..4; this restatement
  * uses the flag for all five variables.
+ *
+ * Pinned by closed forms and invariants (tests/test_oracle_pins.py,
+ * tests/test_oracle_pins_nonlinear.py): the linear schemes and closures,
+ * the nonlinear reconstructions' ENO behaviour at a step and the MP limiter's
+ * extremum preservation, Steger-Warming, SSP-RK3, the viscous operator and
+ * heat flux, metrics, boundary conditions, filter stencil and sensor, the
+ * characteristic projection, the Q criterion.  Parity unpinned (restated
+ * from the source and checked against the HIP path only): the WENO7 and
+ * WENO7-SYMBO smoothness-indicator coefficients themselves (their effect at
+ * a step is pinned, their values are not), the SYMBO TV limiter
+ * (WENO_TV_Limiter, WENO_TV_MAX), the rand()-seeded IBC 108 and IBC 124 wall
+ * blowing tables, the flag of hybrid variables 1..4 above, the filter's
+ * coverage geometry (which rows the reference's launch shapes reach), the
+ * IBC 108 top-face start x_begin, and which node-to-node difference the
+ * CHARTERIC boundary-node rules take.
  */
 #include "ocfd_oracle.h"
 
