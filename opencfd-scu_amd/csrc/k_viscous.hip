@@ -63,6 +63,10 @@ constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
 template <typename S, bool DIV = false> struct KZ_ { static constexpr int v = DIV ? OCFD_VDIV_KZ_D : OCFD_VISC_KZ_D; };
 template <bool DIV> struct KZ_<float, DIV> { static constexpr int v = DIV ? OCFD_VDIV_KZ_F : OCFD_VISC_KZ_F; };
 constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
+#ifndef OCFD_VISC_RING
+#define OCFD_VISC_RING 1  // FP64 gradient kernel: halo strips fetched five planes ahead into an LDS ring
+                          // (the FP32 form would lose its third wave per SIMD to the registers)
+#endif
 #ifndef OCFD_VISC_SKIP_CORNERS
 #define OCFD_VISC_SKIP_CORNERS 1  // the gradient kernel does not load the tile's halo corners
 #endif
@@ -257,6 +261,93 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
         for (int f = 0; f < 4; f++) q[f][m] = src[f][col + (long long)(t.kbeg - 4 + m) * g.sz];
 #pragma unroll
     for (int f = 0; f < 4; f++) nq[f] = src[f][col + (long long)(t.kbeg + 4) * g.sz];
+    if constexpr (OCFD_VISC_RING && std::is_same<S, double>::value) {
+    // The halo strips of plane k+5 are fetched in step k, when the
+    // neighbouring tiles fetch the same rows as their own z-queue plane k+5
+    // (so one of the two reads hits L2), and wait in an LDS ring of RING
+    // planes until step k+5.  Each thread keeps its own halo positions (a
+    // dense numbering h of the strips: x strips first, then y strips), so
+    // the ring needs no barrier of its own.
+    constexpr int NHALO = 8 * TY + 8 * TX, HPT = (NHALO + NT - 1) / NT, RING = 5;
+    __shared__ S ring[RING][4][NHALO];
+    int hpos[HPT];
+    long long hoff[HPT];
+#pragma unroll
+    for (int r = 0; r < HPT; r++) {
+        const int h = tid + NT * r;
+        int hx = 0, hy = 0;
+        if (h < 8 * TY) {
+            const int c = h % 8;
+            hx = c < 4 ? c : TX + c;
+            hy = 4 + h / 8;
+        } else {
+            const int h2 = h - 8 * TY, rw = h2 / TX;
+            hx = 4 + h2 % TX;
+            hy = rw < 4 ? rw : TY + rw;
+        }
+        const int gi = i0 - 4 + hx, gj = j0 - 4 + hy;
+        hpos[r] = h < NHALO ? hy * PW + hx : -1;
+        hoff[r] = (h < NHALO && gi <= g.nx + 3 && gj <= g.ny + 3) ? gidx(g, gi, gj, 0) : -1;
+    }
+    S hn[HPT][4];
+    auto load_strips = [&](int k) {
+        const long long pk = (long long)k * g.sz;
+#pragma unroll
+        for (int r = 0; r < HPT; r++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) hn[r][f] = (hoff[r] >= 0 && k < t.kend) ? src[f][hoff[r] + pk] : S(0);
+    };
+    auto store_strips = [&](int k) {
+        const int sl = (k - t.kbeg) % RING;
+#pragma unroll
+        for (int r = 0; r < HPT; r++)
+            if (hpos[r] >= 0)
+#pragma unroll
+                for (int f = 0; f < 4; f++) ring[sl][f][tid + NT * r] = hn[r][f];
+    };
+#pragma unroll
+    for (int p = 0; p < RING; p++) {
+        load_strips(t.kbeg + p);
+        store_strips(t.kbeg + p);
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f++) cv[f] = co >= 0 ? src[f][co + (long long)t.kbeg * g.sz] : S(0);
+    for (int k = t.kbeg; k < t.kend; k++) {
+        const int sl = (k - t.kbeg) % RING;
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            q[f][8] = nq[f];
+            pl[f][(ty + 4) * PW + tx + 4] = t.own ? q[f][4] : cv[f];
+#pragma unroll
+            for (int r = 0; r < HPT; r++)
+                if (hpos[r] >= 0) pl[f][hpos[r]] = ring[sl][f][tid + NT * r];
+        }
+        __syncthreads();
+        if (k + 1 < t.kend) {
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                nq[f] = src[f][col + (long long)(k + 5) * g.sz];
+                cv[f] = co >= 0 ? src[f][co + (long long)(k + 1) * g.sz] : S(0);
+            }
+            load_strips(k + 5);
+        }
+        if (t.own) {
+            if (near_closure(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+            else gf_point<SV, S, PW, false>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+        }
+        // plane k+5 into the slot plane k has just left
+        if (k + 1 < t.kend) store_strips(k + 5);
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+#pragma unroll
+            for (int m = 0; m < 8; m++) q[f][m] = q[f][m + 1];
+        __syncthreads();
+    }
+    (void)sl;
+    (void)so;
+    (void)hv;
+    (void)load_plane;
+    } else {
     load_plane(t.kbeg);
     for (int k = t.kbeg; k < t.kend; k++) {
         if (!VISC_PREFETCH && k > t.kbeg) {
@@ -287,6 +378,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
 #pragma unroll
             for (int m = 0; m < 8; m++) q[f][m] = q[f][m + 1];
         __syncthreads();
+    }
     }
 }
 
@@ -471,11 +563,13 @@ void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
     if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
     else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
 }
-// Flux components per divergence launch: one on the FP64 path (a quarter of
-// the registers: 4 waves per SIMD instead of 2, measured faster), all four on
-// the FP32 path.
+// Flux components per divergence launch: two on the FP64 path (measured on
+// the 1024x384x256 swept ramp: 3.00-3.30 ms per stage for two launches of
+// two against 3.34-3.59 ms for four of one and 3.27-3.44 ms for one of four;
+// on the 512x128x64 ramp four of one were the fastest), all four on the FP32
+// path.
 #ifndef OCFD_VDIV_NC_D
-#define OCFD_VDIV_NC_D 1  // flux components per divergence launch, FP64 path
+#define OCFD_VDIV_NC_D 2  // flux components per divergence launch, FP64 path
 #endif
 template <typename S> struct NC_ { static constexpr int v = OCFD_VDIV_NC_D; };
 #ifndef OCFD_VDIV_NC_F
