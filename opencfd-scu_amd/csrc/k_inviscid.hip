@@ -976,18 +976,22 @@ void launch_char_scheme(int dir, const SweepArgsT<S>& a, bool first, hipStream_t
 #ifndef OCFD_LMARCH
 #define OCFD_LMARCH 32  // output nodes per marching chunk (k_march)
 #endif
-// y/z tile shapes (output nodes x columns): rows of 64 bytes suit both
-// precisions (8 doubles, 16 floats).  Measured on the 512x128x64 ramp (FP64):
-// NS = 32 with 320 threads (no partial tile on lines of 64 and 128 nodes) is
-// 10-20 % slower than NS = 28 with 256 threads; a per-block queue that hands
+// y/z tile shapes (output nodes x columns).  Measured on the 512x128x64 ramp
+// (FP64, 28 x 8 tiles): NS = 32 with 320 threads (no partial tile on lines
+// of 64 and 128 nodes) is 10-20 % slower than NS = 28 with 256 threads; a
+// per-block queue that hands
 // the rare WENO faces of hybrid_auto to all threads, and a form with the face
 // pass fused into the node pass by cross-lane shuffles (one more resident
 // wave per SIMD), were no faster.
+// FP64: 14 nodes x 16 columns (128-byte row segments): on the 1024x384x256
+// swept ramp the y sweep 3.58-3.74 -> 3.43-3.53 ms and the z sweep
+// 3.91-4.00 -> 3.53-3.58 ms against 28 x 8 (12 x 16 in between; 56 x 4, 4.4
+// and 6.2 ms), although it splits 1.57 nodes per output node against 1.29.
 #ifndef OCFD_YZ_NS_D
-#define OCFD_YZ_NS_D 28
+#define OCFD_YZ_NS_D 14
 #endif
 #ifndef OCFD_YZ_NA_D
-#define OCFD_YZ_NA_D 8
+#define OCFD_YZ_NA_D 16
 #endif
 #ifndef OCFD_YZ_NS_F
 #define OCFD_YZ_NS_F 15
