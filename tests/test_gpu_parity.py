@@ -311,6 +311,40 @@ def test_jacobi_writer_round_trip(tmp_path):
     assert n == arr[0].size
 
 
+def test_jacobi_file_layout_and_oracle_metrics(tmp_path):
+    """OCFD3d-Jacobi.dat parsed byte by byte as the reference reads it
+    (src/OCFD_Comput_Jacobian3d.cu:148-215: 13 arrays Axx, Ayy, Azz, Akx ..
+    Asz, Ajac, each NZ Fortran records int32 n | n bytes | int32 n of NX*NY
+    float64, x fastest), on a curvilinear closed box whose metrics use the
+    derivative closures: the coordinates as loaded, the ten metric arrays
+    equal to the oracle's from the same mesh to 1e-13, nothing after them."""
+    import struct
+    case = cases.wavy_box(20, 16, 12, scheme="WENO7", periodic=(0, 0, 0))
+    nx, ny, nz = case.cfg.nx, case.cfg.ny, case.cfg.nz
+    a = case.load(capi.Solver(case.cfg))
+    path = tmp_path / "OCFD3d-Jacobi.dat"
+    a.write_jacobi(path)
+    raw = path.read_bytes()
+    off, arrays = 0, []
+    for _ in range(13):
+        planes = []
+        for _ in range(nz):
+            (nb,) = struct.unpack_from("<i", raw, off)
+            assert nb == nx * ny * 8
+            planes.append(np.frombuffer(raw, "<f8", nx * ny, off + 4).reshape(ny, nx))
+            (nb2,) = struct.unpack_from("<i", raw, off + 4 + nb)
+            assert nb2 == nb
+            off += nb + 8
+        arrays.append(np.stack(planes))
+    assert off == len(raw)
+    for m in range(3):
+        assert np.array_equal(arrays[m], case.mesh[m])
+    om = case.load(O.Oracle(case.cfg)).get_metrics()
+    for m in range(3, 13):
+        e = np.abs(arrays[m] - om[m]).max() / np.abs(om[m]).max()
+        assert e <= 1e-13, (m, e)
+
+
 def test_step_diagnostics():
     """Means of f[4] and T over the interior, NaN / negative counts of T over
     the field with its ghost layers (the KSTEP_SHOW report)."""
