@@ -153,7 +153,8 @@ def mixed_leg(case, solver64, args, value64, sensor=None):
     if sensor is not None:
         flags = m.get_flags()
         n = sum(f.size for f in flags)
-        out["sensor"] = "learned (BF16-MFMA MLP, confidence band [0.2, 0.8])"
+        out["sensor"] = ("learned (BF16-MFMA MLP inside the x sweep of the first RK stage, "
+                         "confidence band [0.2, 0.8])")
         out["weno_fraction"] = round(sum(int((f > 1).sum()) for f in flags) / n, 4)
     m.close()
     return out

@@ -26,6 +26,7 @@
 // with the accumulation into du are always double.
 #include <type_traits>
 
+#include "mlp_dev.h"
 #include "ocfd_dev.h"
 #include "ocfd_kernels.h"
 
@@ -357,7 +358,12 @@ __device__ __forceinline__ int nth_lane(unsigned long long mask, int n) {
 #endif
 constexpr int XS = 56, XA = OCFD_XA, XN = XS + 8, XF = XS + 1;
 
-template <int SCH, bool FIRST, typename S>
+// LEARN (first RK stage with the learned sensor, hybrid_auto): before phase
+// 1 the wave of a row runs the learned sensor on its 64 nodes s0-4..s0+59
+// (features, two BF16-MFMA layers, the flag rule; mlp_dev.h), writes the
+// flags and probabilities of the nodes its faces use (s0-1..s0+55) and takes
+// each face's flag from the lane of its node instead of loading it.
+template <int SCH, bool FIRST, typename S, bool LEARN = false>
 __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     static_assert(XF <= 64 && XN == 64, "one row per wave; one node and one face per thread");
     __shared__ S sh[10 * XN * XA];
@@ -381,7 +387,38 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     const int jf = s0 - 1 + f_loc, acf = a0 + a_loc;
     const bool fvalid = fthread && (jf <= n_dir - 1) && (acf < g.ny);
     int hy = 0;
-    if (SCH == OCFD_HYBRIDAUTO && fvalid) hy = A.flag[gidx(g, jf, acf, k2)];
+    if constexpr (LEARN) {
+        static_assert(SCH == OCFD_HYBRIDAUTO, "the learned sensor selects hybrid_auto's schemes");
+        __shared__ unsigned short sXm[XA][64 * mlp::LS];
+        __shared__ float sZm[XA][64];
+        mlp::FeatIn<S> F;
+        F.g = g;
+        F.d = A.d; F.u = A.u; F.v = A.v; F.w = A.w; F.T = A.T;
+        for (int m = 0; m < M_COUNT; m++) F.M[m] = A.mlp.M[m];
+        F.p00 = A.c.p00;
+        F.rama = A.c.rama;
+        mlp::Lane L;
+        mlp::lane_weights_packed(A.mlp.P, lane, L);
+        const int sn = s0 - 4 + lane;  // this lane's node
+        const int ic = sn < -1 ? -1 : (sn > n_dir - 1 ? n_dir - 1 : sn);
+        const int jr = acf < g.ny ? acf : g.ny - 1;
+        float f[mlp::NF];
+        mlp::features(F, ic, jr, k2, f);
+        const float prob = mlp::wave_prob(f, sXm[wrow], sZm[wrow], L, lane);
+        const long long qn = gidx(g, ic, jr, k2);
+        const int fl = mlp::flag_rule(prob, mlp::jameson_kp(F, qn, A.c.hy_i1, A.c.hy_i2), A.mlp.lo, A.mlp.hi);
+        if (lane >= 3 && lane <= 3 + XS && sn == ic && acf < g.ny) {
+            A.mlp.fx[qn] = (int8_t)fl;
+            if (sn >= 0) {
+                A.mlp.fy[qn] = (int8_t)fl;
+                A.mlp.fz[qn] = (int8_t)fl;
+            }
+            if (A.mlp.prob) A.mlp.prob[qn] = prob;
+        }
+        hy = __shfl(fl, f_loc + 3);  // face jf+1/2 has the flag of node jf = s0-4 + (f_loc+3)
+    } else {
+        if (SCH == OCFD_HYBRIDAUTO && fvalid) hy = A.flag[gidx(g, jf, acf, k2)];
+    }
     const bool nthread = lane < XS;
     const int n3 = nthread ? lane : 0, a3 = wrow;
     const int i3 = s0 + n3, ac3 = a0 + a3;
@@ -1011,6 +1048,12 @@ void launch_dir(const SweepArgsT<S>& a, bool first, hipStream_t st) {
     const Geo& g = a.g;
     if constexpr (DIR == 0) {
         dim3 grid((g.nx + XS - 1) / XS, (g.ny + XA - 1) / XA, g.nz);
+        if constexpr (SCH == OCFD_HYBRIDAUTO) {
+            if (a.mlp.P && first) {
+                hipLaunchKernelGGL((k_sweep<SCH, true, S, true>), grid, dim3(64 * XA), 0, st, a);
+                return;
+            }
+        }
         if (first) hipLaunchKernelGGL((k_sweep<SCH, true, S>), grid, dim3(64 * XA), 0, st, a);
         else hipLaunchKernelGGL((k_sweep<SCH, false, S>), grid, dim3(64 * XA), 0, st, a);
     } else {

@@ -84,8 +84,10 @@ struct ocfd_ctx {
     double *ub1 = nullptr, *fxd = nullptr, *gzd = nullptr;
     float* mlp_params = nullptr;  // learned sensor (null: the plain sensor)
     float* mlp_store = nullptr;   // device copy of the parameters
+    float* mlp_packed = nullptr;  // and the inference kernels' packed form of them
     float* mlp_prob = nullptr;    // its probability per node (LAP layout)
     float mlp_lo = 0.2f, mlp_hi = 0.8f;
+    int mlp_fused = -1;           // learned sensor in the x sweep (1), its own pass (0), by precision (-1)
     double* diag_part = nullptr;  // 2 * diag_sum_blocks partial sums
     int* diag_scan = nullptr;     // 2 * DIAG_SCAN_BLOCKS counts
     double* b124 = nullptr;       // IBC 124 tables: inlet [5][nz][ny], upper [5][ny][nx], coeff [3][ny][nx], udist [ny][nx]
@@ -421,7 +423,7 @@ MlpSensorArgsT<S> mlp_args(ocfd_ctx* x) {
     a.c = x->c;
     a.d = V.d; a.u = V.u; a.v = V.v; a.w = V.w; a.T = V.T;
     for (int m = 0; m < M_COUNT; m++) a.M[m] = x->M[m];
-    a.params = x->mlp_params;
+    a.params = x->mlp_packed;
     a.lo = x->mlp_lo;
     a.hi = x->mlp_hi;
     a.fx = x->flag[0]; a.fy = x->flag[1]; a.fz = x->flag[2];
@@ -433,15 +435,34 @@ template <typename S>
 int rhs(ocfd_ctx* x, int krk) {
     const ocfd_config& C = x->cfg;
     const View<S> V = view<S>(x);
-    if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1) {
-        if (x->mlp_params) {
-            const MlpSensorArgsT<S> ma = mlp_args<S>(x);
-            TRY(timed(x, "sensor_mlp", [&] { launch_mlp_sensor(ma, x->st); }));
+    // hybrid_auto flags, once per step at the first stage: the plain sensor
+    // before the periodic fill, as du_comput orders it (it reads the ghost
+    // layers the previous stage's fill left); the learned sensor, which has
+    // no counterpart in the reference, after the fill, from the filled ghost
+    // layers, either in its own pass over all nodes or by the x sweep for
+    // the nodes of its rows (i >= -1, j, k >= 0) and here for the ghost
+    // planes j = -1 and k = -1 that only the y and z sweeps read.  Both forms
+    // run the same arithmetic (mlp_dev.h) and give the same flags; by default
+    // the x sweep runs it on the mixed path (7.23 ms against 5.80 + 1.60 ms
+    // for the pass and the plain x sweep on the 1024x384x256 swept ramp) and
+    // its own pass on the FP64 path (where the fused kernel's registers cost
+    // more: 15.8 ms against 9.85 + 2.25 ms); ocfd_set_sensor_fused chooses.
+    // With CHARTERIC = 1, whose sweeps are other kernels, the separate pass.
+    const bool learned = C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1 && x->mlp_params;
+    const bool fuse_pref = x->mlp_fused < 0 ? C.precision == OCFD_PREC_MIXED : x->mlp_fused != 0;
+    const bool fuse_learned = learned && fuse_pref && !C.charteric;
+    if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1 && !learned)
+        TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
+    TRY(fill_prims<S>(x));
+    const MlpSensorArgsT<S> ma = mlp_args<S>(x);
+    if (learned) {
+        if (!fuse_learned) {
+            TRY(timed(x, "sensor_mlp", [&] { launch_mlp_sensor(ma, 0, x->st); }));
         } else {
-            TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
+            TRY(timed(x, "sensor_mlp_planes", [&] { launch_mlp_sensor(ma, 1, x->st); }));
+            TRY(timed(x, "sensor_mlp_planes", [&] { launch_mlp_sensor(ma, 2, x->st); }));
         }
     }
-    TRY(fill_prims<S>(x));
     ViscArgsT<S> va;
     va.g = x->g;
     va.c = x->c;
@@ -481,11 +502,18 @@ int rhs(ocfd_ctx* x, int krk) {
         a.h = dir == 0 ? x->g.hx : dir == 1 ? x->g.hy : x->g.hz;
         a.bl = C.dif_bound[2 * dir]; a.br = C.dif_bound[2 * dir + 1];
         a.nrp = C.non_ref[2 * dir]; a.nrm = C.non_ref[2 * dir + 1];
+        const bool fuse = fuse_learned && dir == 0;
+        for (int m = 0; m < M_COUNT; m++) a.mlp.M[m] = x->M[m];
+        a.mlp.P = fuse ? ma.params : nullptr;
+        a.mlp.lo = ma.lo; a.mlp.hi = ma.hi;
+        a.mlp.fx = ma.fx; a.mlp.fy = ma.fy; a.mlp.fz = ma.fz;
+        a.mlp.prob = ma.prob;
         int lrc = 0;
         if (C.charteric)
             TRY(timed(x, names[dir], [&] { lrc = launch_sweep_char(C.scheme_invis, dir, a, dir == 0, x->st); }));
         else
-            TRY(timed(x, names[dir], [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
+            TRY(timed(x, fuse ? "sweep_x_learned" : names[dir],
+                      [&] { lrc = launch_sweep(C.scheme_invis, dir, a, dir == 0, x->st); }));
         if (lrc) return fail(OCFD_ERR_UNSUPPORTED, "scheme not supported");
     }
     if (overlap) {
@@ -1400,14 +1428,26 @@ int ocfd_set_sensor(ocfd_ctx* x, const float* params, float lo, float hi) {
         if (rc) return rc;
         float* pp = (float*)alloc_bytes(x, sizeof(float) * MLP_NPARAM, rc);
         if (rc) return rc;
+        float* pk = (float*)alloc_bytes(x, sizeof(float) * MLP_PACKED_FLOATS, rc);
+        if (rc) return rc;
         x->mlp_prob = pb;
         x->mlp_store = pp;
+        x->mlp_packed = pk;
     }
     HIPCK(hipMemcpyAsync(x->mlp_store, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, x->st));
+    launch_mlp_pack(x->mlp_store, x->mlp_packed, x->st);
+    HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(x->st));
     x->mlp_params = x->mlp_store;
     x->mlp_lo = lo;
     x->mlp_hi = hi;
+    return 0;
+}
+
+int ocfd_set_sensor_fused(ocfd_ctx* x, int on) {
+    if (!x) return fail(OCFD_ERR_ARG, "null context");
+    if (on < -1 || on > 1) return fail(OCFD_ERR_ARG, "ocfd_set_sensor_fused takes -1, 0 or 1");
+    x->mlp_fused = on;
     return 0;
 }
 

@@ -23,6 +23,17 @@ struct SweepArgsT {
     double h;                            // hx / hy / hz
     int bl, br;                          // DIF_BOUND on the low / high face
     int nrp, nrm;                        // NON_REFLETION for the P / M flux
+    // The learned hybrid sensor, run by the x sweep of the first RK stage
+    // (hybrid_auto only): when P is set the x sweep computes the flags of its
+    // rows' nodes (i >= -1) itself, writes them to fx, fy, fz and the
+    // probabilities to prob, and uses them for its own faces.
+    struct {
+        const double* M[M_COUNT];        // cofactors and Ajac (the features' contravariant velocities)
+        const float* P;                  // packed network weights (launch_mlp_pack), or null
+        float lo, hi;                    // confidence band that falls back to the plain sensor
+        int8_t *fx, *fy, *fz;
+        float* prob;
+    } mlp;
 };
 template <typename S>
 int launch_sweep(int scheme, int dir, const SweepArgsT<S>& a, bool first, hipStream_t st);
@@ -204,18 +215,24 @@ void launch_time_avg(const Geo& g, const S* d, const S* u, const S* v, const S* 
 // Learned sensor (k_mlp.hip): parameters W1 [32][24], b1, W2 [32][32], b2,
 // W3, b3 as 1889 floats.
 constexpr int MLP_NPARAM = 1889;
+// The weights as the inference kernels read them (launch_mlp_pack): the
+// BF16 MFMA fragments and biases of each of a wave's 64 lanes, lane fastest.
+constexpr int MLP_PACKED_FLOATS = 1412;
+void launch_mlp_pack(const float* params, float* packed, hipStream_t st);
 template <typename S>
 struct MlpSensorArgsT {
     Geo g;
     Consts c;
     const S *d, *u, *v, *w, *T;
     const double *M[M_COUNT];
-    const float *params;
+    const float *params;          // packed weights (launch_mlp_pack)
     float lo, hi;                 // confidence band that falls back to the plain sensor
     int8_t *fx, *fy, *fz;
     float *prob;                  // probability per node (LAP layout) or null
 };
-template <typename S> void launch_mlp_sensor(const MlpSensorArgsT<S>& a, hipStream_t st);
+// plane 0: the whole pass; 1 / 2: only the ghost plane j = -1 / k = -1 (the
+// rest is computed by the x sweep of the first RK stage)
+template <typename S> void launch_mlp_sensor(const MlpSensorArgsT<S>& a, int plane, hipStream_t st);
 // Features and plain labels of the interior nodes (i, j, k) with i, j, k
 // multiples of stride, in lattice order: mlp_collect_count(g, stride) rows.
 template <typename S> void launch_mlp_collect(const MlpSensorArgsT<S>& a, int stride, float* X, float* y, hipStream_t st);

@@ -185,6 +185,7 @@ def load_library(path: os.PathLike | None = None) -> C.CDLL:
         "ocfd_get_metrics": (C.c_int, [vp, dp]),
         "ocfd_write_jacobi": (C.c_int, [vp, C.c_char_p]),
         "ocfd_set_sensor": (C.c_int, [vp, C.c_void_p, C.c_float, C.c_float]),
+        "ocfd_set_sensor_fused": (C.c_int, [vp, C.c_int]),
         "ocfd_get_sensor_prob": (C.c_int, [vp, fp]),
         "ocfd_mlp_create": (C.c_int, [C.c_int, C.c_longlong, C.POINTER(C.c_void_p)]),
         "ocfd_mlp_destroy": (C.c_int, [vp]),
@@ -490,6 +491,13 @@ class Solver:
         p = np.ascontiguousarray(params, dtype=np.float32)
         assert p.size == MLP_NPARAM
         _check(self.lib.ocfd_set_sensor(self.h, p.ctypes.data_as(C.c_void_p), lo, hi), "set_sensor")
+
+    def set_sensor_fused(self, on=True):
+        """Run the learned sensor inside the x sweep of the first stage (True),
+        in its own pass (False), or where it is faster for the precision
+        (None: the x sweep on the mixed path, the default); the same flags
+        either way."""
+        _check(self.lib.ocfd_set_sensor_fused(self.h, -1 if on is None else (1 if on else 0)), "set_sensor_fused")
 
     def get_sensor_prob(self):
         out = np.empty(self.nx * self.ny * self.nz, np.float32)

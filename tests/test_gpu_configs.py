@@ -161,11 +161,25 @@ def shipped_params():
     return np.load(Path(pyocfd.__file__).parent / "sensor_params.npy")
 
 
-def test_c4_learned_inference_on_a_ramp():
+def fill_periodic_z(a, nz):
+    """Periodic z ghosts of a LAP box (nz+8, ny+8, nx+8) over the interior x, y
+    columns, as the library's face fill writes them (faces only)."""
+    a = a.copy()
+    a[0:4, 4:-4, 4:-4] = a[nz:nz + 4, 4:-4, 4:-4]
+    a[nz + 4:nz + 8, 4:-4, 4:-4] = a[4:8, 4:-4, 4:-4]
+    return a
+
+
+@pytest.mark.parametrize("fused", [False, True], ids=["own-pass", "in-x-sweep"])
+def test_c4_learned_inference_on_a_ramp(fused):
     """The shipped network on a 64x40x12 ramp (IBC 108 ghosts, closures,
-    periodic z): probabilities against the NumPy forward on features from
-    the oracle's LAP fields (ghost layers as the reference leaves them), and
-    the flags by the flag rule."""
+    periodic z) after three steps, in its own pass and inside the x sweep:
+    probabilities against the NumPy forward on features from the oracle's
+    LAP fields, and every flag the sweeps read by the flag rule - the
+    interior, the column i = -1 and the ghost planes j = -1 and k = -1.  The
+    learned sensor reads the ghost layers after the stage's periodic fill,
+    so the oracle's z ghosts (left by the previous stage's fill) are filled
+    from its interior first."""
     case = cases.ramp(64, 40, 12, ma=2.9, angle=24.0, nsteps=6, filt=False)
     params = shipped_params()
     g = case.load(capi.Solver(case.cfg))
@@ -174,30 +188,77 @@ def test_c4_learned_inference_on_a_ramp():
     o.step(3)
     assert (relerr(g.get_state(), o.get_state()) <= TOL).all()
     g.set_sensor(params, 0.2, 0.8)
+    g.set_sensor_fused(fused)
     g.rhs(1)
     prob = g.get_sensor_prob().reshape(-1)
-    d, u, v, w, T = [o.get_prim_lap(k) for k in range(5)]
+    nz, ny, nx = 12, 40, 64
+    d, u, v, w, T = [fill_periodic_z(o.get_prim_lap(k), nz) for k in range(5)]
     A = np.stack([o.get_metric_lap(3 + m) for m in range(9)])
     p00 = 1.0 / (case.cfg.gamma * case.cfg.ama ** 2)
-    nz, ny, nx = 12, 40, 64
+    P = p00 * d * T
+    pm = capi.mlp_unpack(params)
     nodes = [(i, j, k) for k in range(nz) for j in range(ny) for i in range(nx)]
     F = M.features_field(d, u, v, w, T, A, p00, 1.0 / case.cfg.ama, nodes).astype(np.float32)
-    ref = mlp_prob(capi.mlp_unpack(params), F)
+    ref = mlp_prob(pm, F)
     assert_prob_close("C4 ramp inference", prob, ref)
-    P = p00 * d * T
     kp = M.jameson_kp(P[3:-3, 3:-3, 3:-3], *case.cfg.hy_dp_intv).reshape(-1)
     want = M.flag_rule(prob, kp, 0.2, 0.8).reshape(nz, ny, nx)
     fx, fy, fz = g.get_flags()
     assert np.array_equal(fx[:, :, 1:], want)
     assert np.array_equal(fy[:, 1:, :], want)
     assert np.array_equal(fz[1:, :, :], want)
+    # the column i = -1 and the planes j = -1, k = -1: flags from the NumPy
+    # probabilities, away from the band edges (within 2e-3 of lo or hi a BF16
+    # rounding may tip the rule)
+    for tag, got, nodes in (
+            ("i=-1", fx[:, :, 0], [(-1, j, k) for k in range(nz) for j in range(ny)]),
+            ("j=-1", fy[:, 0, :], [(i, -1, k) for k in range(nz) for i in range(nx)]),
+            ("k=-1", fz[0, :, :], [(i, j, -1) for j in range(ny) for i in range(nx)])):
+        Fg = M.features_field(d, u, v, w, T, A, p00, 1.0 / case.cfg.ama, nodes).astype(np.float32)
+        rg = mlp_prob(pm, Fg)
+        kg = np.array([M.jameson_kp(P[k + 3:k + 6, j + 3:j + 6, i + 3:i + 6], *case.cfg.hy_dp_intv).item()
+                       for (i, j, k) in nodes])
+        wg = M.flag_rule(rg, kg, 0.2, 0.8)
+        sure = (np.abs(rg - 0.2) > 2e-3) & (np.abs(rg - 0.8) > 2e-3)
+        print(f"[configs] C4 ramp flags at {tag}: {int(sure.sum())} of {sure.size} nodes compared", flush=True)
+        assert np.array_equal(got.reshape(-1)[sure], wg[sure]), tag
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_learned_sensor_in_x_sweep_equals_own_pass(prec):
+    """The learned sensor inside the x sweep of the first stage and in its
+    own pass run the same arithmetic: the same flags and probabilities bit
+    for bit on a swept ramp after three steps, and the same state bit for bit
+    after five more steps each way (captured graphs included)."""
+    case = cases.ramp(72, 40, 16, sweep=30.0, nsteps=12, filt=False)
+    cfg = capi.OcfdConfig.from_buffer_copy(case.cfg)
+    cfg.precision = prec
+    params = shipped_params()
+    a = case.load(capi.Solver(cfg))
+    b = case.load(capi.Solver(cfg))
+    for s, fused in ((a, False), (b, True)):
+        s.step(3)
+        s.set_sensor(params, 0.2, 0.8)
+        s.set_sensor_fused(fused)
+        s.rhs(1)
+    for fa, fb in zip(a.get_flags(), b.get_flags()):
+        assert np.array_equal(fa, fb)
+    assert np.array_equal(a.get_sensor_prob(), b.get_sensor_prob())
+    a.step(5)
+    b.step(5)
+    sa, sb = a.get_state(), b.get_state()
+    assert all(np.array_equal(sa[v], sb[v]) for v in range(5))
+    weno = sum(int((f > 1).sum()) for f in a.get_flags())
+    print(f"[configs] learned sensor in the x sweep = own pass ({'fp64' if prec == capi.PREC_FP64 else 'mixed'}), "
+          f"{weno} WENO faces", flush=True)
+    assert weno > 0
 
 
 def test_c4_full_size_learned_inference():
     """1024x256x128 (C4's grid) with the shipped weights: two steps, then
-    probabilities on a seeded sample of interior nodes (at least 4 nodes
-    from every face: the sensor reads the ghost layers before this stage's
-    periodic fill, so they hold the previous stage's values) against the NumPy forward on
+    probabilities on a seeded sample of interior nodes at least 4 nodes from
+    every face (the sampled stencils stay clear of the ghost layers, which
+    the downloaded primitives do not hold) against the NumPy forward on
     features from the GPU's own primitives, and the learned flags against
     the plain sensor's."""
     case = cases.ramp(1024, 256, 128, nsteps=4)
