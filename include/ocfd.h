@@ -263,13 +263,11 @@ int ocfd_hybrid_proportions(ocfd_ctx *ctx, double frac9[9]);
  * params = NULL switches back to the plain sensor. */
 int ocfd_set_sensor(ocfd_ctx *ctx, const float *params, float lo, float hi);
 /* Where the learned sensor runs (no counterpart in the reference), after the
- * first stage's periodic fill: on = 1 inside the x sweep of the first stage
- * (the BF16-MFMA forward of the 64 nodes of each x-sweep wave, with the
- * ghost planes j = -1 and k = -1 in a pass of their own); on = 0 in its own
- * pass over all nodes; on = -1 (the default) the faster of the two as
- * measured: the x sweep on the mixed path, its own pass on the FP64 path.
- * The same flags and probabilities either way.  CHARTERIC = 1 always uses
- * the separate pass. */
+ * first stage's periodic fill: on != 0 (the default) inside the x sweep of
+ * the first stage (the BF16-MFMA forward of the 64 nodes of each x-sweep
+ * wave, with the ghost planes j = -1 and k = -1 in a pass of their own);
+ * on = 0 in its own pass over all nodes.  The same flags and probabilities
+ * either way.  CHARTERIC = 1 always uses the separate pass. */
 int ocfd_set_sensor_fused(ocfd_ctx *ctx, int on);
 /* Interior probabilities of the last learned-sensor pass (nz*ny*nx floats). */
 int ocfd_get_sensor_prob(ocfd_ctx *ctx, float *prob);

@@ -390,7 +390,6 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     if constexpr (LEARN) {
         static_assert(SCH == OCFD_HYBRIDAUTO, "the learned sensor selects hybrid_auto's schemes");
         __shared__ unsigned short sXm[XA][64 * mlp::LS];
-        __shared__ float sZm[XA][64];
         mlp::FeatIn<S> F;
         F.g = g;
         F.d = A.d; F.u = A.u; F.v = A.v; F.w = A.w; F.T = A.T;
@@ -404,7 +403,7 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
         const int jr = acf < g.ny ? acf : g.ny - 1;
         float f[mlp::NF];
         mlp::features(F, ic, jr, k2, f);
-        const float prob = mlp::wave_prob(f, sXm[wrow], sZm[wrow], L, lane);
+        const float prob = mlp::wave_prob(f, sXm[wrow], L, lane);
         const long long qn = gidx(g, ic, jr, k2);
         const int fl = mlp::flag_rule(prob, mlp::jameson_kp(F, qn, A.c.hy_i1, A.c.hy_i2), A.mlp.lo, A.mlp.hi);
         if (lane >= 3 && lane <= 3 + XS && sn == ic && acf < g.ny) {

@@ -67,7 +67,6 @@ __global__ void __launch_bounds__(256) k_mlp_sensor(FeatIn<S> F, const float* __
                                                     float lo, float hi, int8_t* fx, int8_t* fy, int8_t* fz,
                                                     float* prob_out, int plane) {
     __shared__ unsigned short sX[4][64 * LS];
-    __shared__ float sZ[4][64];
     const Geo& g = F.g;
     const int lane = threadIdx.x, wv = threadIdx.y;
     Lane L;
@@ -88,7 +87,7 @@ __global__ void __launch_bounds__(256) k_mlp_sensor(FeatIn<S> F, const float* __
     const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1), kc = min(k, g.nz - 1);
     float f[NF];
     features(F, ic, jc, kc, f);
-    const float prob = wave_prob(f, sX[wv], sZ[wv], L, lane);
+    const float prob = wave_prob(f, sX[wv], L, lane);
     if (!own) return;
     const long long q = gidx(g, i, j, k);
     const int fl = flag_rule(prob, jameson_kp(F, q, i1, i2), lo, hi);

@@ -133,13 +133,21 @@ __device__ __forceinline__ unsigned short to_bf16(float f) { return __builtin_bi
 
 constexpr int LS = 36;  // LDS row stride (bf16 elements) of a wave's 64 x 32 activation tile
 
-// The weights a lane feeds the MFMAs, in the B-operand layout of
-// v_mfma_f32_16x16x16bf16: w1[kk][o] = W1[16o + lane%16][16kk + 4(lane/16)
-// .. +3] in BF16 (W1 padded from 24 to 32 inputs with zeros), w2 the same
-// for W2; b1, b2, w3 the entries 16o + lane%16 of b1, b2 and W3.
+// Each layer is computed transposed, Z^T = W . X^T, with the weights as the
+// A operand and the activations as the B operand of
+// v_mfma_f32_16x16x16bf16 (A: lane l holds A[l%16][4(l/16) .. +3]; B: lane l
+// holds B[4(l/16) .. +3][l%16]; D: lane l holds D[4(l/16) .. +3][l%16]).  The
+// output of a layer then sits in each lane exactly as the next layer's B
+// operand needs it (node 16t + l%16, hidden units 16o + 4(l/16) .. +3), so
+// only the input features pass through LDS.
+//
+// The weights a lane feeds: w1[kk][o] = W1[16o + lane%16][16kk + 4(lane/16)
+// .. +3] in BF16 (W1 padded from 24 to 32 inputs with zeros), w2 the same for
+// W2; b1[o], b2[o], w3[o] = the entries 16o + 4(lane/16) .. +3 of b1, b2, W3.
 struct Lane {
     v4s w1[2][2], w2[2][2];
-    float b1[2], b2[2], w3[2], b3;
+    v4f b1[2], b2[2], w3[2];
+    float b3;
 };
 __device__ __forceinline__ void lane_weights(const float* __restrict__ P, int lane, Lane& L) {
     const int r = lane & 15, g4 = 4 * (lane >> 4);
@@ -154,19 +162,21 @@ __device__ __forceinline__ void lane_weights(const float* __restrict__ P, int la
                 L.w2[kk][o][e] = (short)to_bf16(P[OW2 + row * NH + col]);
             }
 #pragma unroll
-    for (int o = 0; o < 2; o++) {
-        L.b1[o] = P[OB1 + 16 * o + r];
-        L.b2[o] = P[OB2 + 16 * o + r];
-        L.w3[o] = P[OW3 + 16 * o + r];
-    }
+    for (int o = 0; o < 2; o++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            L.b1[o][e] = P[OB1 + 16 * o + g4 + e];
+            L.b2[o][e] = P[OB2 + 16 * o + g4 + e];
+            L.w3[o][e] = P[OW3 + 16 * o + g4 + e];
+        }
     L.b3 = P[OB3];
 }
 
 // The same fragments packed once per parameter set (k_mlp_pack), lane
 // fastest, so that a wave reads each with one coalesced access: w1 and w2
 // as [kk][o][lane] groups of 4 BF16 (2 floats of storage), b1, b2, w3 as
-// [o][lane], then b3.
-constexpr int PK_W1 = 0, PK_W2 = 512, PK_B1 = 1024, PK_B2 = 1152, PK_W3 = 1280, PK_B3 = 1408;
+// [o][lane] groups of 4 floats, then b3.
+constexpr int PK_W1 = 0, PK_W2 = 512, PK_B1 = 1024, PK_B2 = 1536, PK_W3 = 2048, PK_B3 = 2560;
 static_assert(PK_B3 + 1 <= MLP_PACKED_FLOATS, "packed weight layout");
 __device__ __forceinline__ void pack_lane(const Lane& L, int lane, float* __restrict__ pk) {
 #pragma unroll
@@ -178,9 +188,9 @@ __device__ __forceinline__ void pack_lane(const Lane& L, int lane, float* __rest
         }
 #pragma unroll
     for (int o = 0; o < 2; o++) {
-        pk[PK_B1 + 64 * o + lane] = L.b1[o];
-        pk[PK_B2 + 64 * o + lane] = L.b2[o];
-        pk[PK_W3 + 64 * o + lane] = L.w3[o];
+        *(v4f*)&pk[PK_B1 + 4 * (64 * o + lane)] = L.b1[o];
+        *(v4f*)&pk[PK_B2 + 4 * (64 * o + lane)] = L.b2[o];
+        *(v4f*)&pk[PK_W3 + 4 * (64 * o + lane)] = L.w3[o];
     }
     if (lane == 0) pk[PK_B3] = L.b3;
 }
@@ -194,75 +204,72 @@ __device__ __forceinline__ void lane_weights_packed(const float* __restrict__ pk
         }
 #pragma unroll
     for (int o = 0; o < 2; o++) {
-        L.b1[o] = pk[PK_B1 + 64 * o + lane];
-        L.b2[o] = pk[PK_B2 + 64 * o + lane];
-        L.w3[o] = pk[PK_W3 + 64 * o + lane];
+        L.b1[o] = *(const v4f*)&pk[PK_B1 + 4 * (64 * o + lane)];
+        L.b2[o] = *(const v4f*)&pk[PK_B2 + 4 * (64 * o + lane)];
+        L.w3[o] = *(const v4f*)&pk[PK_W3 + 4 * (64 * o + lane)];
     }
     L.b3 = pk[PK_B3];
 }
 
-// Z(64 x 32) = X(64 x 32) . W^T for one wave: X rows are nodes (bf16, row
-// stride LS).  acc[t][o] element e holds Z[16t + 4*(lane/16) + e][16o + lane%16].
-__device__ __forceinline__ void wave_layer(const unsigned short* X, const v4s (&w)[2][2], int lane, v4f acc[4][2]) {
-    const int r = lane & 15, g4 = 4 * (lane >> 4);
+// One layer for the wave's 64 nodes: acc[t][o] element e = Z[node 16t +
+// lane%16][out 16o + 4(lane/16) + e] from the B fragments x[t][kk] = X[node
+// 16t + lane%16][16kk + 4(lane/16) .. +3].
+__device__ __forceinline__ void wave_layer(const v4s (&w)[2][2], const v4s (&x)[4][2], v4f acc[4][2]) {
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int o = 0; o < 2; o++) acc[t][o] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kk = 0; kk < 2; kk++) {
+    for (int kk = 0; kk < 2; kk++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const v4s a = *(const v4s*)&X[(16 * t + r) * LS + 16 * kk + g4];
+        for (int t = 0; t < 4; t++)
 #pragma unroll
             for (int o = 0; o < 2; o++)
-                acc[t][o] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w[kk][o], acc[t][o], 0, 0, 0);
-        }
-    }
+                acc[t][o] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w[kk][o], x[t][kk], acc[t][o], 0, 0, 0);
 }
 
 // Probability of the node of each lane of the wave.  f = the lane's 24
-// features; X = the wave's 64 x LS tile and Z = 64 floats, both in LDS.  Every
-// thread of the block calls it (it separates its phases with block barriers).
-__device__ __forceinline__ float wave_prob(const float* f, unsigned short* X, float* Z, const Lane& L, int lane) {
+// features; X = the wave's 64 x LS tile in LDS, through which the features
+// reach the lanes that feed them to the first layer.  Every thread of the
+// block calls it (one block barrier).
+__device__ __forceinline__ float wave_prob(const float* f, unsigned short* X, const Lane& L, int lane) {
 #pragma unroll
     for (int e = 0; e < 32; e++) X[lane * LS + e] = to_bf16(e < NF ? f[e] : 0.f);
     __syncthreads();
-    v4f acc[4][2];
-    wave_layer(X, L.w1, lane, acc);
-    __syncthreads();
     const int r = lane & 15, g4 = 4 * (lane >> 4);
+    v4s x[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) x[t][kk] = *(const v4s*)&X[(16 * t + r) * LS + 16 * kk + g4];
+    v4f acc[4][2];
+    wave_layer(L.w1, x, acc);
+    // bias + GELU, rounded to BF16: the second layer's B fragments
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int o = 0; o < 2; o++)
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                X[(16 * t + g4 + e) * LS + 16 * o + r] = to_bf16(gelu_fast(acc[t][o][e] + L.b1[o]));
-    __syncthreads();
-    wave_layer(X, L.w2, lane, acc);
-    // output layer: per node sum over the 32 hidden units (16 lanes x 2 tiles)
-    float z[4][4];
+            for (int e = 0; e < 4; e++) x[t][o][e] = (short)to_bf16(gelu_fast(acc[t][o][e] + L.b1[o][e]));
+    wave_layer(L.w2, x, acc);
+    // output layer: the lane's 8 hidden units of node 16t + lane%16, then the
+    // sum over the 4 lanes that hold that node's other units
+    float z[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < 4; t++) {
+        float s = 0.f;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            float s = 0.f;
+        for (int o = 0; o < 2; o++)
 #pragma unroll
-            for (int o = 0; o < 2; o++) s += gelu_fast(acc[t][o][e] + L.b2[o]) * L.w3[o];
-            s += __shfl_xor(s, 8);
-            s += __shfl_xor(s, 4);
-            s += __shfl_xor(s, 2);
-            s += __shfl_xor(s, 1);
-            z[t][e] = s;
-        }
-    if (r == 0)
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) Z[16 * t + g4 + e] = z[t][e];
-    __syncthreads();
-    return 1.f / (1.f + expf(-(Z[lane] + L.b3)));
+            for (int e = 0; e < 4; e++) s += gelu_fast(acc[t][o][e] + L.b2[o][e]) * L.w3[o][e];
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        z[t] = s;
+    }
+    // this lane's node is 16 (lane/16) + lane%16
+    const int tl = lane >> 4;
+    const float zl = tl == 0 ? z[0] : tl == 1 ? z[1] : tl == 2 ? z[2] : z[3];
+    return 1.f / (1.f + expf(-(zl + L.b3)));
 }
 
 }  // namespace mlp

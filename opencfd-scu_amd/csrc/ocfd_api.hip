@@ -87,7 +87,7 @@ struct ocfd_ctx {
     float* mlp_packed = nullptr;  // and the inference kernels' packed form of them
     float* mlp_prob = nullptr;    // its probability per node (LAP layout)
     float mlp_lo = 0.2f, mlp_hi = 0.8f;
-    int mlp_fused = -1;           // learned sensor in the x sweep (1), its own pass (0), by precision (-1)
+    bool mlp_fused = true;        // learned sensor in the x sweep (else its own pass)
     double* diag_part = nullptr;  // 2 * diag_sum_blocks partial sums
     int* diag_scan = nullptr;     // 2 * DIAG_SCAN_BLOCKS counts
     double* b124 = nullptr;       // IBC 124 tables: inlet [5][nz][ny], upper [5][ny][nx], coeff [3][ny][nx], udist [ny][nx]
@@ -441,16 +441,12 @@ int rhs(ocfd_ctx* x, int krk) {
     // no counterpart in the reference, after the fill, from the filled ghost
     // layers, either in its own pass over all nodes or by the x sweep for
     // the nodes of its rows (i >= -1, j, k >= 0) and here for the ghost
-    // planes j = -1 and k = -1 that only the y and z sweeps read.  Both forms
-    // run the same arithmetic (mlp_dev.h) and give the same flags; by default
-    // the x sweep runs it on the mixed path (7.23 ms against 5.80 + 1.60 ms
-    // for the pass and the plain x sweep on the 1024x384x256 swept ramp) and
-    // its own pass on the FP64 path (where the fused kernel's registers cost
-    // more: 15.8 ms against 9.85 + 2.25 ms); ocfd_set_sensor_fused chooses.
-    // With CHARTERIC = 1, whose sweeps are other kernels, the separate pass.
+    // planes j = -1 and k = -1 that only the y and z sweeps read (the
+    // default; ocfd_set_sensor_fused chooses).  Both forms run the same
+    // arithmetic (mlp_dev.h) and give the same flags.  With CHARTERIC = 1,
+    // whose sweeps are other kernels, the separate pass.
     const bool learned = C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1 && x->mlp_params;
-    const bool fuse_pref = x->mlp_fused < 0 ? C.precision == OCFD_PREC_MIXED : x->mlp_fused != 0;
-    const bool fuse_learned = learned && fuse_pref && !C.charteric;
+    const bool fuse_learned = learned && x->mlp_fused && !C.charteric;
     if (C.scheme_invis == OCFD_HYBRIDAUTO && krk == 1 && !learned)
         TRY(timed(x, "sensor", [&] { launch_sensor(x->g, x->c, V.d, V.T, x->flag[0], x->flag[1], x->flag[2], x->st); }));
     TRY(fill_prims<S>(x));
@@ -1446,8 +1442,7 @@ int ocfd_set_sensor(ocfd_ctx* x, const float* params, float lo, float hi) {
 
 int ocfd_set_sensor_fused(ocfd_ctx* x, int on) {
     if (!x) return fail(OCFD_ERR_ARG, "null context");
-    if (on < -1 || on > 1) return fail(OCFD_ERR_ARG, "ocfd_set_sensor_fused takes -1, 0 or 1");
-    x->mlp_fused = on;
+    x->mlp_fused = on != 0;
     return 0;
 }
 

@@ -217,7 +217,7 @@ void launch_time_avg(const Geo& g, const S* d, const S* u, const S* v, const S* 
 constexpr int MLP_NPARAM = 1889;
 // The weights as the inference kernels read them (launch_mlp_pack): the
 // BF16 MFMA fragments and biases of each of a wave's 64 lanes, lane fastest.
-constexpr int MLP_PACKED_FLOATS = 1412;
+constexpr int MLP_PACKED_FLOATS = 2564;
 void launch_mlp_pack(const float* params, float* packed, hipStream_t st);
 template <typename S>
 struct MlpSensorArgsT {

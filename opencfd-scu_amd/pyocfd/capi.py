@@ -493,11 +493,9 @@ class Solver:
         _check(self.lib.ocfd_set_sensor(self.h, p.ctypes.data_as(C.c_void_p), lo, hi), "set_sensor")
 
     def set_sensor_fused(self, on=True):
-        """Run the learned sensor inside the x sweep of the first stage (True),
-        in its own pass (False), or where it is faster for the precision
-        (None: the x sweep on the mixed path, the default); the same flags
-        either way."""
-        _check(self.lib.ocfd_set_sensor_fused(self.h, -1 if on is None else (1 if on else 0)), "set_sensor_fused")
+        """Run the learned sensor inside the x sweep of the first stage (True,
+        the default) or in its own pass (False); the same flags either way."""
+        _check(self.lib.ocfd_set_sensor_fused(self.h, 1 if on else 0), "set_sensor_fused")
 
     def get_sensor_prob(self):
         out = np.empty(self.nx * self.ny * self.nz, np.float32)
