@@ -37,9 +37,9 @@ ALGO_BYTES = {
     "sweep_x": (5 + 3 + 1 + 5) * 8,       # rho,u,v,w,T + 3 cofactors + Ajac; write du(5)
     "sweep_y": (5 + 3 + 1 + 5 + 5) * 8,   # same + read-modify-write du(5)
     "sweep_z": (5 + 3 + 1 + 5 + 5) * 8,
-    "visc_flux": (4 + 1 + 10 + 12) * 8,   # u,v,w,T, mu, 9 cofactors + Ajac; write 12 Ev
+    "visc_flux": (4 + 10 + 12) * 8,       # u,v,w,T, 9 cofactors + Ajac; write 12 Ev (mu formed from T)
     "visc_div": (2 * 3 + 1 + 2) * 8,      # two flux components per launch: their 6 Ev, Ajac; write 2 dv
-    "stage": (5 + 4 + 5 + 5 + 5 + 5 + 1) * 8,  # du, dv, fn, f; write f, 5 primitives, mu (stages 2-3; p in 3)
+    "stage": (5 + 4 + 5 + 5 + 5 + 5) * 8,  # du, dv, fn, f; write f, 5 primitives (stages 2-3; p in 3)
     "sensor": 2 * 8 + 3,                  # rho, T; write 3 int8 flags
 }
 

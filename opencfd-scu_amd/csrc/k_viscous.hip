@@ -131,19 +131,24 @@ template <int SV, typename S, int PW, bool CL>
 __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k, const S (*pl)[PW * (TY + 8)],
                                          const S (*q)[9], int tx, int ty, S rhx, S rhy, S rhz) {
     const Geo& g = A.g;
-    // The point's metrics and viscosity are loaded first, so that the loads
-    // are in flight while the derivatives are formed.  They are read once
-    // here and the fluxes are written once: all go through the caches as
-    // streaming (non-temporal) accesses, which leaves L2 to the plane strips
-    // that neighbouring tiles share (measured 203 -> 196 us, and the
-    // divergence that reads Ev next 42 -> 40 us).
+    // The point's metrics are loaded first, so that the loads are in flight
+    // while the derivatives are formed.  They are read once here and the
+    // fluxes are written once: all go through the caches as streaming
+    // (non-temporal) accesses, which leaves L2 to the plane strips that
+    // neighbouring tiles share (measured 203 -> 196 us, and the divergence
+    // that reads Ev next 42 -> 40 us).
     const long long qi = gidx(g, t.i, t.j, k), fs = A.fs;
     auto ld = [&](int m) { return S(__builtin_nontemporal_load(&A.M[m][qi])); };
     const S Akx = ld(M_KX), Aky = ld(M_KY), Akz = ld(M_KZ);
     const S Aix = ld(M_IX), Aiy = ld(M_IY), Aiz = ld(M_IZ);
     const S Asx = ld(M_SX), Asy = ld(M_SY), Asz = ld(M_SZ);
-    const S Amu = __builtin_nontemporal_load(&A.amu[qi]);
     const S Aj1 = ld(M_JAC);
+    // Sutherland's law at the point (get_Amu_kernal, src/commen_kernel.cu:
+    // 122-131) from its T (the centre of the z queue), in FP64 as the RK stage
+    // and the boundary kernels form it: the same value as a viscosity field
+    // written after each stage, without its 8 B/pt store and reload
+    const double Tp = double(q[3][4]);
+    const S Amu = S(A.c.amu_c0 * sqrt(Tp * Tp * Tp) / (A.c.tsb + Tp));
     S gr[4][3];
 #pragma unroll
     for (int f = 0; f < 4; f++) {

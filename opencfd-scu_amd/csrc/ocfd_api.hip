@@ -596,7 +596,9 @@ int one_step_t(ocfd_ctx* x) {
     std::swap(x->f, x->fn);
     const bool fdue = x->flap && filter_due(x, x->istep + 1);
     for (int krk = 1; krk <= 3; krk++) {
-        const StageArgsT<S> sa = stage_args<S>(x, krk, krk == 3 && fdue);
+        // no viscosity field: the gradient kernel forms mu from T at its points
+        StageArgsT<S> sa = stage_args<S>(x, krk, krk == 3 && fdue);
+        sa.amu = nullptr;
         TRY(rhs<S>(x, krk));
         TRY(timed(x, "stage", [&] { launch_stage(sa, x->st); }));
         TRY(apply_bc<S>(x));
@@ -1327,7 +1329,16 @@ int ocfd_get_pressure(ocfd_ctx* x, double* p) {
 
 int ocfd_get_amu(ocfd_ctx* x, double* a) {
     if (!x || !a) return fail(OCFD_ERR_ARG, "null argument");
-    return x->mixed ? download_any(x, x->samu, a) : download_any(x, x->amu, a);
+    // the time step does not keep the viscosity field (the gradient kernel
+    // forms mu from T): get_Amu's kernel fills it from the current T first
+    if (x->mixed) {
+        const BCArgsT<float> b = bc_args<float>(x);
+        TRY(timed(x, "amu", [&] { launch_amu_all(b, x->st); }));
+        return download_any(x, x->samu, a);
+    }
+    const BCArgsT<double> b = bc_args<double>(x);
+    TRY(timed(x, "amu", [&] { launch_amu_all(b, x->st); }));
+    return download_any(x, x->amu, a);
 }
 
 int ocfd_get_metrics(ocfd_ctx* x, double* a13) {

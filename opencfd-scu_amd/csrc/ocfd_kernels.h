@@ -85,7 +85,8 @@ struct StageArgsT {
     const S *du;
     const S *dv;                             // 4 fields, the viscous part of du[1..4], or null
     double *flap;                            // 5 fields or null (written in stage 3 when non-null)
-    S *d, *u, *v, *w, *T, *amu;
+    S *d, *u, *v, *w, *T;
+    S *amu;                                  // viscosity, or null (the stage then skips it)
     S *P;                                    // pressure, or null (the stage then skips it)
     long long fs;
 };
@@ -128,10 +129,11 @@ __device__ __forceinline__ void stage_point(const StageArgsT<S>& A, long long q,
     const double u1 = fv[1] / d1, v1 = fv[2] / d1, w1 = fv[3] / d1;
     const double T2 = (fv[4] - 0.5 * d1 * (u1 * u1 + v1 * v1 + w1 * w1)) / (d1 * A.c.cv);
     A.d[q] = S(d1); A.u[q] = S(u1); A.v[q] = S(v1); A.w[q] = S(w1); A.T[q] = S(T2);
-    // the pressure (read after the step) and the viscosity (read by the
-    // gradient kernel alone) are streamed: 2.857 -> 2.848 ms/step
+    // the pressure (read after the step) is streamed; the viscosity only
+    // when amu is set (the time step leaves it null: the gradient kernel forms
+    // the viscosity from T itself)
     if (A.P) __builtin_nontemporal_store(S(T2 * d1 / (A.c.gamma * A.c.ama * A.c.ama)), &A.P[q]);
-    __builtin_nontemporal_store(S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2)), &A.amu[q]);
+    if (A.amu) __builtin_nontemporal_store(S(A.c.amu_c0 * sqrt(T2 * T2 * T2) / (A.c.tsb + T2)), &A.amu[q]);
 }
 
 template <typename S> void launch_c2p_all(const StageArgsT<S>& a, hipStream_t st);
