@@ -34,12 +34,12 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per interior point per launch: each array the kernel
 # must read or write, once (FP64 = 8 B).
 ALGO_BYTES = {
-    "sweep_x": (5 + 3 + 1 + 5) * 8,       # rho,u,v,w,T + 3 cofactors + Ajac; write du(5)
-    "sweep_y": (5 + 3 + 1 + 5 + 5) * 8,   # same + read-modify-write du(5)
-    "sweep_z": (5 + 3 + 1 + 5 + 5) * 8,
+    "sweep_x": (5 + 3 + 5) * 8,           # rho,u,v,w,T + 3 cofactors; write du(5) (Ajac applied by the stage)
+    "sweep_y": (5 + 3 + 5 + 5) * 8,       # same + read-modify-write du(5)
+    "sweep_z": (5 + 3 + 5 + 5) * 8,
     "visc_flux": (4 + 10 + 12) * 8,       # u,v,w,T, 9 cofactors + Ajac; write 12 Ev (mu formed from T)
-    "visc_div": (2 * 3 + 1 + 2) * 8,      # two flux components per launch: their 6 Ev, Ajac; write 2 dv
-    "stage": (5 + 4 + 5 + 5 + 5 + 5) * 8,  # du, dv, fn, f; write f, 5 primitives (stages 2-3; p in 3)
+    "visc_div": (2 * 3 + 2) * 8,          # two flux components per launch: their 6 Ev; write 2 dv
+    "stage": (5 + 4 + 1 + 5 + 5 + 5 + 5) * 8,  # du, dv, Ajac, fn, f; write f, 5 primitives (stages 2-3; p in 3)
     "sensor": 2 * 8 + 3,                  # rho, T; write 3 int8 flags
 }
 

@@ -13,9 +13,11 @@
 //            stage them in LDS (they never touch HBM);
 //   phase 2  one thread per half node: P and M reconstructions of all 5
 //            variables with the reference's boundary closures;
-//   phase 3  one thread per node: du += -Ajac*(hP(i+1/2)-hP(i-1/2))/h, then
-//            += the M term, in the reference's order (x P, x M, y P, ...), with
-//            no atomics.
+//   phase 3  one thread per node: du += -(hP(i+1/2)-hP(i-1/2))/h, then += the
+//            M term, in the reference's order (x P, x M, y P, ...), with no
+//            atomics.  The reference multiplies every term by the node's Ajac;
+//            here the RK stage multiplies their sum once (k_stage), so the
+//            sweeps do not read Ajac.
 //
 // y and z (k_tile): the same three phases on tiles of x-consecutive columns;
 // k_march (the same steps per node while marching along the sweep, with the
@@ -379,8 +381,8 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     // phases 2 and 3: wave a handles row a, lane = position along the row
     // (57 faces, 56 nodes), so that the LDS accesses of a wave are
     // consecutive doubles of one row (no bank conflicts).  The hybrid flag of
-    // this thread's face and the Ajac of its output node are fetched first,
-    // so that their latency hides behind phase 1
+    // this thread's face is fetched first, so that its latency hides behind
+    // phase 1
     const int lane = tid & 63, wrow = tid >> 6;
     const bool fthread = lane < XF;
     const int f_loc = fthread ? lane : 0, a_loc = wrow;
@@ -423,7 +425,6 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     const int i3 = s0 + n3, ac3 = a0 + a3;
     const bool nvalid = nthread && i3 < n_dir && ac3 < g.ny;
     const long long q3 = nvalid ? gidx(g, i3, ac3, k2) : 0;
-    const double ajac3 = nvalid ? A.jac[q3] : 0.0;
 
     // ---- phase 1: split of XN x XA nodes into LDS
     {
@@ -517,12 +518,11 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
     __syncthreads();
 
     // ---- phase 3: du at nodes, in the reference's order P then M; the
-    // Jacobian factor and the accumulation are in double
+    // accumulation is in double
     if (nvalid) {
         const int n = n3, a = a3, i = i3;
         {
             const long long q = q3;
-            const double ajac = ajac3;
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
             const double rh = 1.0 / A.h;
@@ -530,8 +530,8 @@ __global__ void __launch_bounds__(64 * XA) k_sweep(SweepArgsT<S> A) {
             for (int m = 0; m < 5; m++) {
                 const S pr = sh[(m * XA + a) * XF + n + 1], pl = sh[(m * XA + a) * XF + n];
                 const S mr = sh[((5 + m) * XA + a) * XF + n + 1], ml = sh[((5 + m) * XA + a) * XF + n];
-                const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
-                const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
+                const double tP = zp ? 0.0 : -double(pr - pl) * rh;
+                const double tM = zm ? 0.0 : -double(mr - ml) * rh;
                 S* dv = A.du + (long long)m * A.fs;
                 double acc = FIRST ? 0.0 : double(dv[q]);
                 acc = acc + tP;
@@ -568,10 +568,10 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
     auto lin = [&](int s, int a) -> long long { return DIR == 1 ? gidx(g, a, s, k2) : gidx(g, a, k2, s); };
     auto at = [](int v, int n, int a) { return (v * NN + n) * NA + a; };
 
-    // the du and Ajac of this thread's output nodes are fetched first, so
-    // that their latency hides behind phases 1 and 2
+    // the du of this thread's output nodes is fetched first, so that its
+    // latency hides behind phases 1 and 2
     constexpr int NPT = (NS * NA + 255) / 256;
-    double pre_du[NPT][5], pre_jac[NPT];
+    double pre_du[NPT][5];
 #pragma unroll
     for (int t = 0; t < NPT; t++) {
         const int idx = tid + 256 * t;
@@ -579,7 +579,6 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
         const int i = s0 + n, ac = a0 + a;
         if (idx < NS * NA && i < n_dir && ac < g.nx) {
             const long long q = lin(i, ac);
-            pre_jac[t] = A.jac[q];
 #pragma unroll
             for (int m = 0; m < 5; m++) pre_du[t][m] = double(A.du[(long long)m * A.fs + q]);
         }
@@ -650,7 +649,7 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
     }
     __syncthreads();
 
-    // ---- phase 3: du at nodes (P then M; Jacobian factor and sum in double)
+    // ---- phase 3: du at nodes (P then M; the sum in double)
     const double rh = 1.0 / A.h;
 #pragma unroll
     for (int t = 0; t < NPT; t++) {
@@ -659,15 +658,14 @@ __global__ void __launch_bounds__(256) TILE_ATTR k_tile(SweepArgsT<S> A) {
         const int i = s0 + n, ac = a0 + a;
         if (idx < NS * NA && i < n_dir && ac < g.nx) {
             const long long q = lin(i, ac);
-            const double ajac = pre_jac[t];
             const bool zp = A.nrp && i == 0;
             const bool zm = A.nrm && i == n_dir - 1;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 const S pr = sh[(m * NF + n + 1) * NA + a], pl = sh[(m * NF + n) * NA + a];
                 const S mr = sh[((5 + m) * NF + n + 1) * NA + a], ml = sh[((5 + m) * NF + n) * NA + a];
-                const double tP = zp ? 0.0 : -ajac * double(pr - pl) * rh;
-                const double tM = zm ? 0.0 : -ajac * double(mr - ml) * rh;
+                const double tP = zp ? 0.0 : -double(pr - pl) * rh;
+                const double tM = zm ? 0.0 : -double(mr - ml) * rh;
                 double acc = pre_du[t][m];
                 acc = acc + tP;
                 acc = acc + tM;
@@ -740,13 +738,12 @@ __global__ void __launch_bounds__(256) k_march(SweepArgsT<S> A) {
                                     hp[m], hm[m]);
         if (j >= s0 && own) {
             const long long q = base + (long long)j * st;
-            const double ajac = A.jac[q];
             const bool zp = A.nrp && j == 0;
             const bool zm = A.nrm && j == n_dir - 1;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
-                const double tP = zp ? 0.0 : -ajac * double(hp[m] - pp[m]) * rh;
-                const double tM = zm ? 0.0 : -ajac * double(hm[m] - pm[m]) * rh;
+                const double tP = zp ? 0.0 : -double(hp[m] - pp[m]) * rh;
+                const double tM = zm ? 0.0 : -double(hm[m] - pm[m]) * rh;
                 S* dv = A.du + (long long)m * A.fs;
                 double acc = double(dv[q]);
                 acc = acc + tP;
@@ -967,19 +964,18 @@ __global__ void __launch_bounds__(256) k_char(SweepArgsT<S> A) {
         const int i = s0 + n, ai = a0 + a;
         if (i < n_dir && ai < n_acr) {
             const long long q = lin(i, ai);
-            const double ajac = A.jac[q];
             const double rh = 1.0 / A.h;
             const bool lowP = fyc == 1 && i == 0, highP = fyc == 2 && i == n_dir - 1;
             const bool highM = fyc == 2 && i == n_dir - 1, lowM = fyc == 1 && i == 0 && DIR != 1;
 #pragma unroll
             for (int m = 0; m < 5; m++) {
                 double tP, tM;
-                if (lowP) tP = A.nrp ? 0.0 : -ajac * double(shb[0][m][a]) * rh;
-                else if (highP) tP = -ajac * double(shb[1][m][a]) * rh;
-                else tP = -ajac * double(sh[fat(m, n + 1, a)] - sh[fat(m, n, a)]) * rh;
-                if (highM) tM = A.nrm ? 0.0 : -ajac * double(shb[1][5 + m][a]) * rh;
-                else if (lowM) tM = -ajac * double(shb[0][5 + m][a]) * rh;
-                else tM = -ajac * double(sh[fat(5 + m, n + 1, a)] - sh[fat(5 + m, n, a)]) * rh;
+                if (lowP) tP = A.nrp ? 0.0 : -double(shb[0][m][a]) * rh;
+                else if (highP) tP = -double(shb[1][m][a]) * rh;
+                else tP = -double(sh[fat(m, n + 1, a)] - sh[fat(m, n, a)]) * rh;
+                if (highM) tM = A.nrm ? 0.0 : -double(shb[1][5 + m][a]) * rh;
+                else if (lowM) tM = -double(shb[0][5 + m][a]) * rh;
+                else tM = -double(sh[fat(5 + m, n + 1, a)] - sh[fat(5 + m, n, a)]) * rh;
                 S* dv = A.du + (long long)m * A.fs;
                 double acc = FIRST ? 0.0 : double(dv[q]);
                 acc = acc + tP;

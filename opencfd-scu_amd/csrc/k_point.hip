@@ -54,6 +54,11 @@ __global__ void __launch_bounds__(256) k_stage(StageArgsT<S> A) {
 #pragma unroll
         for (int m = 1; m < 5; m++) du[m] += double(__builtin_nontemporal_load(&A.dv[(m - 1) * A.fs + q]));
     }
+    // the sweeps and the divergence leave out the node's Ajac, by which the
+    // reference multiplies each of their terms: it is applied to the sum here
+    const double ajac = __builtin_nontemporal_load(&A.jac[q]);
+#pragma unroll
+    for (int m = 0; m < 5; m++) du[m] *= ajac;
     stage_point(A, q, du);
 }
 

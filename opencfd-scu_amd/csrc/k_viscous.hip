@@ -472,7 +472,6 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
         auto point = [&](auto cl) {
             constexpr bool CL = decltype(cl)::value;
             const long long qi = gidx(g, t.i, t.j, k);
-            const double ajac = A.M[M_JAC][qi];
 #pragma unroll
             for (int m = 0; m < NC; m++) {
                 S s[9];
@@ -483,18 +482,20 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
                 for (int r = 0; r < 9; r++) s[r] = ey[m][(ty + r) * TX + tx];
                 const S vy = cd9<SV, S, CL>(s, t.j, g.ny, A.bnd[2], A.bnd[3], rhy);
                 const S vz = cd9<SV, S, CL>(q[m], k, g.nz, A.bnd[4], A.bnd[5], rhz);
-                // the Jacobian factor and the sums are double
+                // the sums are double; the node's Ajac, by which the
+                // reference multiplies each of these terms, is applied to the
+                // whole right-hand side by the RK stage
                 if (SEP) {
-                    double acc = ajac * double(vx);
-                    acc = acc + ajac * double(vy);
-                    acc = acc + ajac * double(vz);
+                    double acc = double(vx);
+                    acc = acc + double(vy);
+                    acc = acc + double(vz);
                     A.dv[(long long)(c0 + m) * fs + qi] = S(acc);
                 } else {
                     S* du = A.du + (long long)(c0 + m + 1) * fs + qi;
                     double acc = double(*du);
-                    acc = acc + ajac * double(vx);
-                    acc = acc + ajac * double(vy);
-                    acc = acc + ajac * double(vz);
+                    acc = acc + double(vx);
+                    acc = acc + double(vy);
+                    acc = acc + double(vz);
                     *du = S(acc);
                 }
             }

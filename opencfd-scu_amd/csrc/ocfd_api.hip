@@ -581,7 +581,7 @@ StageArgsT<S> stage_args(ocfd_ctx* x, int krk, bool write_flap) {
     const View<S> V = view<S>(x);
     StageArgsT<S> sa;
     sa.g = x->g; sa.c = x->c; sa.krk = krk; sa.dt = x->cfg.dt;
-    sa.f = x->f; sa.fn = x->fn; sa.du = V.du; sa.dv = V.dv;
+    sa.f = x->f; sa.fn = x->fn; sa.du = V.du; sa.dv = V.dv; sa.jac = x->M[M_JAC];
     sa.flap = write_flap ? x->flap : nullptr;
     sa.d = V.d; sa.u = V.u; sa.v = V.v; sa.w = V.w; sa.T = V.T; sa.amu = V.amu;
     // the pressure is read only after a step (filter, get_pressure, analysis):
@@ -1311,14 +1311,17 @@ int ocfd_get_du(ocfd_ctx* x, double* du5) {
         TRY(x->mixed ? download_any(x, x->sdu + m * x->fs, du5 + m * n)
                      : download_any(x, x->du + m * x->fs, du5 + m * n));
     // the viscous part of components 1..4 when it is kept apart, added as
-    // the RK stage adds it
+    // the RK stage adds it, then the Jacobian factor the stage applies
+    std::vector<double> tmp(n);
     if (!x->mixed) {
-        std::vector<double> dv(n);
         for (int m = 1; m < 5; m++) {
-            TRY(download_any(x, x->dv + (m - 1) * x->fs, dv.data()));
-            for (size_t p = 0; p < n; p++) du5[m * n + p] += dv[p];
+            TRY(download_any(x, x->dv + (m - 1) * x->fs, tmp.data()));
+            for (size_t p = 0; p < n; p++) du5[m * n + p] += tmp[p];
         }
     }
+    TRY(download_any(x, (const double*)x->M[M_JAC], tmp.data()));
+    for (int m = 0; m < 5; m++)
+        for (size_t p = 0; p < n; p++) du5[m * n + p] *= tmp[p];
     return 0;
 }
 

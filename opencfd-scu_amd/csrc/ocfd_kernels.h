@@ -82,8 +82,9 @@ struct StageArgsT {
     double dt;
     double *f;
     const double *fn;
-    const S *du;
+    const S *du;                             // right-hand side without the Jacobian factor
     const S *dv;                             // 4 fields, the viscous part of du[1..4], or null
+    const double *jac;                       // Ajac: the stage advances with Ajac * (du + dv)
     double *flap;                            // 5 fields or null (written in stage 3 when non-null)
     S *d, *u, *v, *w, *T;
     S *amu;                                  // viscosity, or null (the stage then skips it)
@@ -95,7 +96,7 @@ template <typename S> void launch_stage(const StageArgsT<S>& a, hipStream_t st);
 // SSP-RK3 stage of one point (OCFD_time_advance_ker1/2/3, src/OCFD_time.cu:
 // 44-94) with cons->prim + p (cons_to_pri_kernel, src/commen_kernel.cu:
 // 77-109) and Sutherland's law (get_Amu_kernal, src/commen_kernel.cu:122-131);
-// du[m] is the point's right-hand side.
+// du[m] is the point's right-hand side (with the Jacobian factor applied).
 template <typename S>
 __device__ __forceinline__ void stage_point(const StageArgsT<S>& A, long long q, const double* du) {
     const long long fs = A.fs;
