@@ -569,18 +569,17 @@ void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
     if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
     else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
 }
-// Flux components per divergence launch: two on the FP64 path (measured on
-// the 1024x384x256 swept ramp: 3.00-3.30 ms per stage for two launches of
-// two against 3.34-3.59 ms for four of one and 3.27-3.44 ms for one of four;
-// on the 512x128x64 ramp four of one were the fastest), all four on the FP32
-// path.
+// Flux components per divergence launch: two (measured on the 1024x384x256
+// swept ramp: FP64 3.00-3.30 ms per stage for two launches of two against
+// 3.34-3.59 ms for four of one and 3.27-3.44 ms for one of four; the mixed
+// step 43.5 -> 43.0 ms with two launches of two instead of one of four; on
+// the 512x128x64 ramp four of one had been the fastest).
 #ifndef OCFD_VDIV_NC_D
 #define OCFD_VDIV_NC_D 2  // flux components per divergence launch, FP64 path
 #endif
 template <typename S> struct NC_ { static constexpr int v = OCFD_VDIV_NC_D; };
 #ifndef OCFD_VDIV_NC_F
-#define OCFD_VDIV_NC_F 4  // flux components per divergence launch, FP32 path (measured on the
-                          // 512x128x64 ramp: 4 in one launch 113 us, 2 x 61 us, 4 x 34 us)
+#define OCFD_VDIV_NC_F 2  // flux components per divergence launch, FP32 path
 #endif
 template <> struct NC_<float> { static constexpr int v = OCFD_VDIV_NC_F; };
 template <typename S>
