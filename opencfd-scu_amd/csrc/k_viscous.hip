@@ -23,8 +23,8 @@
 // double.
 //
 // Both use 2.5-D blocking: a block owns a TX x TY tile of (x,y) columns and
-// marches through a chunk of z planes (gradients: 32 on the FP64 path, 16 on
-// the FP32 path; divergence: 16).  x and y neighbours come from the
+// marches through a chunk of z planes (32 on the FP64 path; 16 on the FP32
+// path).  x and y neighbours come from the
 // current plane staged in LDS with its 4-point halo; z neighbours come from a
 // 9-plane queue each thread keeps in registers for its own column.
 #include <type_traits>
@@ -52,14 +52,20 @@ namespace {
 #endif
 constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
 #ifndef OCFD_VDIV_KZ_D
-#define OCFD_VDIV_KZ_D 16  // z planes per block of the divergence, FP64 path
+#define OCFD_VDIV_KZ_D 32  // z planes per block of the divergence, FP64 path: 32 against 16 on the
+                           // 1024x384x256 swept ramp, the divergence's share of the step 13.65 ->
+                           // 13.44 % and the step 64.34 -> 64.17 ms (bench runs alternated); no
+                           // change on the 512x128x64 and 1024x256x128 ramps
 #endif
 #ifndef OCFD_VDIV_KZ_F
 #define OCFD_VDIV_KZ_F 16  // z planes per block of the divergence, FP32 path
 #endif
+#ifndef OCFD_VDIV_TY
+#define OCFD_VDIV_TY 8  // y columns per block of the divergence
+#endif
+constexpr int TYD = OCFD_VDIV_TY;
 // z planes per block: KZ_<S, false> for the gradient/flux kernel, KZ_<S,
-// true> for the divergence (a lighter kernel at higher occupancy, measured
-// faster with more, shorter chunks)
+// true> for the divergence
 template <typename S, bool DIV = false> struct KZ_ { static constexpr int v = DIV ? OCFD_VDIV_KZ_D : OCFD_VISC_KZ_D; };
 template <bool DIV> struct KZ_<float, DIV> { static constexpr int v = DIV ? OCFD_VDIV_KZ_F : OCFD_VISC_KZ_F; };
 constexpr bool VISC_PREFETCH = OCFD_VISC_PREFETCH;
@@ -106,13 +112,13 @@ struct Tile25 {
     int i0, j0, i, j, kbeg, kend;
     bool own;
 };
-template <typename S, bool DIV>
+template <typename S, bool DIV, int TYB = TY>
 __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
     constexpr int KZ = KZ_<S, DIV>::v;
     Tile25 t;
     const Blk B = xcd_block();
     t.i0 = B.x * TX;
-    t.j0 = B.y * TY;
+    t.j0 = B.y * TYB;
     t.i = t.i0 + threadIdx.x;
     t.j = t.j0 + threadIdx.y;
     t.kbeg = B.z * KZ;
@@ -206,11 +212,11 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
 
 // Whether a central-derivative closure is in reach of the block's plane k:
 // the same for every thread of the block.
-template <typename S>
+template <int TYB = TY, typename S>
 __device__ __forceinline__ bool near_closure(const ViscArgsT<S>& A, const Tile25& t, int k) {
     const Geo& g = A.g;
     return (A.bnd[0] && t.i0 <= 3) || (A.bnd[1] && t.i0 + TX > g.nx - 4) || (A.bnd[2] && t.j0 <= 3) ||
-           (A.bnd[3] && t.j0 + TY > g.ny - 4) || (A.bnd[4] && k <= 3) || (A.bnd[5] && k >= g.nz - 4);
+           (A.bnd[3] && t.j0 + TYB > g.ny - 4) || (A.bnd[4] && k <= 3) || (A.bnd[5] && k >= g.nz - 4);
 }
 
 // The plane loads of both kernels go through per-thread slots: the slot's
@@ -391,12 +397,13 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
 // One launch handles NC of the four flux components, starting at c0 (each
 // component is independent; fewer components per launch need fewer
 // registers).
+constexpr int NTD = TX * TYD;
 template <int SV, typename S, int NC, bool SEP>
-__global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
-    constexpr int XW = TX + 8, XN = TY * XW, YN = (TY + 8) * TX;
-    constexpr int NSX = (XN + NT - 1) / NT, NSY = (YN + NT - 1) / NT;
+__global__ void __launch_bounds__(NTD) k_visc_div(ViscArgsT<S> A, int c0) {
+    constexpr int XW = TX + 8, XN = TYD * XW, YN = (TYD + 8) * TX;
+    constexpr int NSX = (XN + NTD - 1) / NTD, NSY = (YN + NTD - 1) / NTD;
     const Geo& g = A.g;
-    const Tile25 t = tile_of<S, true>(g);
+    const Tile25 t = tile_of<S, true, TYD>(g);
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = t.i0, j0 = t.j0;
     const S rhx = S(1.0 / g.hx), rhy = S(1.0 / g.hy), rhz = S(1.0 / g.hz);
@@ -412,7 +419,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     long long sox[NSX], soy[NSY];
 #pragma unroll
     for (int r = 0; r < NSX; r++) {
-        const int p = tid + NT * r;
+        const int p = tid + NTD * r;
         const int hx = p % XW, hy = p / XW;
         const int gi = i0 - 4 + hx, gj = j0 + hy;
         slx[r] = p < XN ? p : -1;
@@ -420,7 +427,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
     }
 #pragma unroll
     for (int r = 0; r < NSY; r++) {
-        const int p = tid + NT * r;
+        const int p = tid + NTD * r;
         const int hx = p % TX, hy = p / TX;
         const int gi = i0 + hx, gj = j0 - 4 + hy;
         sly[r] = p < YN ? p : -1;
@@ -501,7 +508,7 @@ __global__ void __launch_bounds__(NT) k_visc_div(ViscArgsT<S> A, int c0) {
             }
         };
         if (t.own) {
-            if (near_closure(A, t, k)) point(std::true_type{});
+            if (near_closure<TYD>(A, t, k)) point(std::true_type{});
             else point(std::false_type{});
         }
 #pragma unroll
@@ -558,8 +565,8 @@ __global__ void __launch_bounds__(256) k_qcrit(ViscArgsT<S> A, double* __restric
 
 template <typename S, bool DIV>
 dim3 grid25(const Geo& g) {
-    constexpr int KZ = KZ_<S, DIV>::v;
-    return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, (g.nz + KZ - 1) / KZ);
+    constexpr int KZ = KZ_<S, DIV>::v, TYB = DIV ? TYD : TY;
+    return dim3((g.nx + TX - 1) / TX, (g.ny + TYB - 1) / TYB, (g.nz + KZ - 1) / KZ);
 }
 
 }  // namespace
@@ -588,7 +595,7 @@ template <typename S>
 void launch_visc_div(const ViscArgsT<S>& a, int c0, hipStream_t st) {
     constexpr int NC = NC_<S>::v;
     // SEP: the terms go to dv (set) or are added to du (null)
-    const dim3 grid = grid25<S, true>(a.g), blk(TX, TY);
+    const dim3 grid = grid25<S, true>(a.g), blk(TX, TYD);
     if (a.svis == OCFD_CD6) {
         if (a.dv) hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC, true>), grid, blk, 0, st, a, c0);
         else hipLaunchKernelGGL((k_visc_div<OCFD_CD6, S, NC, false>), grid, blk, 0, st, a, c0);
