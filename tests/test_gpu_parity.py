@@ -97,8 +97,8 @@ def test_hybrid_flag_mixes(intv, closed):
 @pytest.mark.parametrize("closed", [False, True])
 def test_hybrid_tall_z(closed):
     """A 20x12x40 wavy box: the sensor's blocks (16 nodes along z each) and
-    the gradient/flux and divergence kernels' blocks (32 and 16 planes on the
-    FP64 path, 8 and 16 on the FP32 path) meet inside the grid, with and
+    the gradient/flux and divergence kernels' blocks (32 planes on the FP64
+    path, 16 on the FP32 path) meet inside the grid, with and
     without closures on every face; the oracle's flags bit for bit and its du
     to the bar."""
     case = cases.wavy_box(20, 12, 40, scheme="SCHEME_HYBRIDAUTO", amp=0.08,
