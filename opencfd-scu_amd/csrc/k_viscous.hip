@@ -35,7 +35,8 @@
 namespace {
 
 #ifndef OCFD_VISC_KZ_D
-#define OCFD_VISC_KZ_D 32  // z planes per block, FP64 path
+#define OCFD_VISC_KZ_D 32  // z planes per block, FP64 path (16 / 24 / 32 / 64: 4.57 / 4.55 / 4.46 /
+                           // 4.74 ms per launch on the 1024x384x256 swept ramp)
 #endif
 #ifndef OCFD_VISC_KZ_F
 #define OCFD_VISC_KZ_F 16  // z planes per block, FP32 path: 8 is the fastest alone, 16 the faster
