@@ -28,10 +28,14 @@
  * the nonlinear reconstructions' ENO behaviour at a step and the MP limiter's
  * extremum preservation, Steger-Warming, SSP-RK3, the viscous operator and
  * heat flux, metrics, boundary conditions, filter stencil and sensor, the
- * characteristic projection, the Q criterion.  Parity unpinned (restated
- * from the source and checked against the HIP path only): the WENO7 and
- * WENO7-SYMBO smoothness-indicator coefficients themselves (their effect at
- * a step is pinned, their values are not), the SYMBO TV limiter
+ * characteristic projection, the Q criterion, and WENO7's face value from
+ * first principles (candidate values of the cell-mean cubics, smoothness
+ * indicators = 34560 x the integrated squared derivatives of the point-value
+ * cubics, optimal weights 1:12:18:4; stencil 2's second-derivative term is
+ * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it).  Parity
+ * unpinned (restated from the source and checked against the HIP path
+ * only): the WENO7-SYMBO smoothness-indicator coefficients themselves (their
+ * effect at a step is pinned, their values are not), the SYMBO TV limiter
  * (WENO_TV_Limiter, WENO_TV_MAX), the rand()-seeded IBC 108 and IBC 124 wall
  * blowing tables, the flag of hybrid variables 1..4 above, the filter's
  * coverage geometry (which rows the reference's launch shapes reach), the

@@ -279,3 +279,118 @@ def test_mp_limiter_is_inactive_on_smooth_monotone_data(scheme):
         h = O.face_flux(capi.SCHEME_IDS[scheme], side, np.exp(x))
         assert abs(h - want) <= 1e-9, (side, h, want)
         assert abs(h - np.exp(x[UPWIND[scheme][side]])) > 1e-3
+
+
+# ---------------------------------------------------------------------------
+# WENO7 smoothness indicators and weights from first principles
+# (src/OCFD_Schemes.cu:616-663).  On the P side the face is x_{i+1/2} and
+# s[0..6] are the nodes i-3..i+3; stencil k (k = 0..3) is nodes i-3+k..i+k.
+# The face value is sum_k w_k q_k with
+#   q_k  the value at the face of the cubic whose means over the cells
+#        [j-1/2, j+1/2] of the stencil are the data (finite-difference WENO),
+#   w_k  = a_k / sum a, a_k = d_k / (eps + beta_k)^2, d = (1, 12, 18, 4) / 35,
+#   beta_k = 34560 * sum_{l=1..3} int_{-1/2}^{1/2} (p_k^(l))^2 dxi for the
+#        cubic p_k through the stencil's point values (the Balsara-Shu
+#        integral with the point-value polynomial; 34560 * 1e-8 = eps =
+#        3.456e-4).
+# The reference's stencil 2 takes its second-derivative term at the node
+# i+1 (its second difference is s[3] - 2 s[4] + s[5] where p_2''(i) is
+# s[2] - 2 s[3] + s[4]); the expected value below does the same, and
+# test_weno7_textbook_stencil_2_differs shows the test tells the two apart.
+# Everything is exact rational arithmetic on the (exact) binary inputs:
+# the polynomials are solved for and integrated here, not transcribed.
+from fractions import Fraction as Fr
+
+
+def _solve(A, b):
+    n = len(A)
+    M = [list(r) + [b[i]] for i, r in enumerate(A)]
+    for c in range(n):
+        p = next(r for r in range(c, n) if M[r][c] != 0)
+        M[c], M[p] = M[p], M[c]
+        for r in range(n):
+            if r != c and M[r][c] != 0:
+                f = M[r][c] / M[c][c]
+                M[r] = [x - f * y for x, y in zip(M[r], M[c])]
+    return [M[i][n] / M[i][i] for i in range(n)]
+
+
+def _cubic(nodes, vals, cell_mean):
+    """Coefficients c0..c3 (in xi, node i at xi = 0, unit spacing) of the
+    cubic with the given point values, or cell means, at the nodes."""
+    def basis(j, e):
+        if not cell_mean:
+            return Fr(j) ** e
+        a, b = Fr(2 * j - 1, 2), Fr(2 * j + 1, 2)
+        return (b ** (e + 1) - a ** (e + 1)) / (e + 1)
+    return _solve([[basis(j, e) for e in range(4)] for j in nodes], vals)
+
+
+def _pmul(a, b):
+    r = [Fr(0)] * (len(a) + len(b) - 1)
+    for i, x in enumerate(a):
+        for j, y in enumerate(b):
+            r[i + j] += x * y
+    return r
+
+
+def _smoothness(c):
+    """sum_{l=1..3} int_{-1/2}^{1/2} (d^l p / dxi^l)^2 dxi."""
+    tot, d = Fr(0), list(c)
+    for _ in range(3):
+        d = [i * d[i] for i in range(1, len(d))]
+        sq = _pmul(d, d)
+        tot += sum(ci * (Fr(1, 2) ** (i + 1) - Fr(-1, 2) ** (i + 1)) / (i + 1) for i, ci in enumerate(sq))
+    return tot
+
+
+def _weno7_face(s, node_of_second=(0, 0, 1, 0)):
+    d = [Fr(1, 35), Fr(12, 35), Fr(18, 35), Fr(4, 35)]
+    eps = Fr(3456, 10 ** 7)
+    num = den = Fr(0)
+    for k in range(4):
+        nodes = [k - 3 + m for m in range(4)]
+        vals = [Fr(float(x)) for x in s[k:k + 4]]
+        cm = _cubic(nodes, vals, True)
+        q = sum(ci * Fr(1, 2) ** e for e, ci in enumerate(cm))
+        c = _cubic(nodes, vals, False)
+        e2 = node_of_second[k]
+        # the same cubic's derivatives, its second derivative taken at xi = e2
+        c_eff = [c[0], c[1], c[2] + 3 * c[3] * e2, c[3]]
+        beta = 34560 * _smoothness(c_eff)
+        a = d[k] / (eps + beta) ** 2
+        num += a * q
+        den += a
+    return num / den
+
+
+def _weno7_inputs():
+    rng = np.random.default_rng(7)
+    out = [rng.uniform(-1.0, 1.0, 8) for _ in range(12)]
+    out += [rng.uniform(0.5, 2.0, 8) * 1e-3 for _ in range(4)]          # small values: eps matters
+    out += [np.sin(0.3 * np.arange(8) + ph) for ph in (0.0, 0.7, 2.1)]  # smooth: near-linear weights
+    out += [np.where(np.arange(8) > m, 2.0, 1.0) + 0.01 * np.arange(8) for m in range(1, 6)]
+    return out
+
+
+@pytest.mark.parametrize("side", [P, M])
+def test_weno7_face_value_from_first_principles(side):
+    """The oracle's WENO7 face value against the candidate values, the
+    integrated smoothness indicators and the optimal weights formed here; M
+    is the P scheme on the mirrored stencil (src/OCFD_Schemes.cu:692-741)."""
+    for s in _weno7_inputs():
+        want = float(_weno7_face(s if side == P else s[6::-1]))
+        got = O.face_flux(capi.SCHEME_IDS["WENO7"], side, s)
+        assert abs(got - want) <= 1e-13 * max(1.0, np.abs(s).max()) + 1e-15, (side, s, got, want)
+
+
+def test_weno7_textbook_stencil_2_differs():
+    """With stencil 2's second derivative at node i (the textbook
+    Balsara-Shu indicator) the expected values move by far more than the
+    tolerance above on rough data: the pin sees where the second derivative
+    of each stencil is taken."""
+    worst = 0.0
+    for s in _weno7_inputs()[:12]:
+        got = O.face_flux(capi.SCHEME_IDS["WENO7"], P, s)
+        worst = max(worst, abs(got - float(_weno7_face(s, (0, 0, 0, 0)))))
+    assert worst > 1e-6, worst
