@@ -32,7 +32,8 @@
  * first principles (candidate values of the cell-mean cubics, smoothness
  * indicators = 34560 x the integrated squared derivatives of the point-value
  * cubics, optimal weights 1:12:18:4; stencil 2's second-derivative term is
- * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it).  Parity
+ * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it), and WENO5's
+ * the same way (Jiang-Shu indicators, weights 1:6:3).  Parity
  * unpinned (restated from the source and checked against the HIP path
  * only): the WENO7-SYMBO smoothness-indicator coefficients themselves (their
  * effect at a step is pinned, their values are not), the SYMBO TV limiter

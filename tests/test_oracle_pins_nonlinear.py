@@ -316,14 +316,15 @@ def _solve(A, b):
 
 
 def _cubic(nodes, vals, cell_mean):
-    """Coefficients c0..c3 (in xi, node i at xi = 0, unit spacing) of the
-    cubic with the given point values, or cell means, at the nodes."""
+    """Coefficients c0..c(n-1) (in xi, node i at xi = 0, unit spacing) of the
+    polynomial of degree n-1 (n nodes) with the given point values, or cell
+    means, at the nodes."""
     def basis(j, e):
         if not cell_mean:
             return Fr(j) ** e
         a, b = Fr(2 * j - 1, 2), Fr(2 * j + 1, 2)
         return (b ** (e + 1) - a ** (e + 1)) / (e + 1)
-    return _solve([[basis(j, e) for e in range(4)] for j in nodes], vals)
+    return _solve([[basis(j, e) for e in range(len(nodes))] for j in nodes], vals)
 
 
 def _pmul(a, b):
@@ -335,9 +336,9 @@ def _pmul(a, b):
 
 
 def _smoothness(c):
-    """sum_{l=1..3} int_{-1/2}^{1/2} (d^l p / dxi^l)^2 dxi."""
+    """sum_{l=1..deg} int_{-1/2}^{1/2} (d^l p / dxi^l)^2 dxi."""
     tot, d = Fr(0), list(c)
-    for _ in range(3):
+    for _ in range(len(c) - 1):
         d = [i * d[i] for i in range(1, len(d))]
         sq = _pmul(d, d)
         tot += sum(ci * (Fr(1, 2) ** (i + 1) - Fr(-1, 2) ** (i + 1)) / (i + 1) for i, ci in enumerate(sq))
@@ -394,3 +395,32 @@ def test_weno7_textbook_stencil_2_differs():
         got = O.face_flux(capi.SCHEME_IDS["WENO7"], P, s)
         worst = max(worst, abs(got - float(_weno7_face(s, (0, 0, 0, 0)))))
     assert worst > 1e-6, worst
+
+
+# WENO5 (Jiang-Shu, src/OCFD_Schemes.cu:1118-1210), the same construction:
+# node i = s[2], face i+1/2, stencils i-2..i, i-1..i+1, i..i+2 with optimal
+# weights 1:6:3 / 10, candidate values of the quadratics whose cell means are
+# the data, indicators 12 * sum_{l=1,2} int (p_k^(l))^2 over the cell (for
+# quadratics the cell-mean and point-value polynomials differ by a constant,
+# so the indicator is the same), eps = 12 * 1e-6.  M is P on s[4::-1].
+def _weno5_face(s):
+    d = [Fr(1, 10), Fr(6, 10), Fr(3, 10)]
+    eps = Fr(12, 10 ** 6)
+    num = den = Fr(0)
+    for k in range(3):
+        nodes = [k - 2 + m for m in range(3)]
+        vals = [Fr(float(x)) for x in s[k:k + 3]]
+        cm = _cubic(nodes, vals, True)
+        q = sum(ci * Fr(1, 2) ** e for e, ci in enumerate(cm))
+        a = d[k] / (eps + 12 * _smoothness(cm)) ** 2
+        num += a * q
+        den += a
+    return num / den
+
+
+@pytest.mark.parametrize("side", [P, M])
+def test_weno5_face_value_from_first_principles(side):
+    for s in _weno7_inputs():
+        want = float(_weno5_face(s if side == P else s[4::-1]))
+        got = O.face_flux(capi.SCHEME_IDS["WENO5"], side, s)
+        assert abs(got - want) <= 1e-13 * max(1.0, np.abs(s).max()) + 1e-15, (side, s, got, want)
