@@ -32,11 +32,12 @@
  * first principles (candidate values of the cell-mean cubics, smoothness
  * indicators = 34560 x the integrated squared derivatives of the point-value
  * cubics, optimal weights 1:12:18:4; stencil 2's second-derivative term is
- * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it), and WENO5's
- * the same way (Jiang-Shu indicators, weights 1:6:3).  Parity
+ * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it), WENO5's the
+ * same way (Jiang-Shu indicators, weights 1:6:3) and WENO7-SYMBO's
+ * nonlinear branch (25920 x the integrals for the cell-mean cubics, the
+ * downwind stencil at the largest indicator, first-power weights).  Parity
  * unpinned (restated from the source and checked against the HIP path
- * only): the WENO7-SYMBO smoothness-indicator coefficients themselves (their
- * effect at a step is pinned, their values are not), the SYMBO TV limiter
+ * only): SYMBO's five optimised weights beyond their sum, the SYMBO TV limiter
  * (WENO_TV_Limiter, WENO_TV_MAX), the rand()-seeded IBC 108 and IBC 124 wall
  * blowing tables, the flag of hybrid variables 1..4 above, the filter's
  * coverage geometry (which rows the reference's launch shapes reach), the

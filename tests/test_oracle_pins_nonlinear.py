@@ -424,3 +424,45 @@ def test_weno5_face_value_from_first_principles(side):
         want = float(_weno5_face(s if side == P else s[4::-1]))
         got = O.face_flux(capi.SCHEME_IDS["WENO5"], side, s)
         assert abs(got - want) <= 1e-13 * max(1.0, np.abs(s).max()) + 1e-15, (side, s, got, want)
+
+
+# WENO7-SYMBO (src/OCFD_Schemes.cu:391-478, the nonlinear branch): five
+# stencils i-3+k..i+k (k = 0..4, the last one downwind of the face), the
+# candidate values of the cubics whose cell means are the data, indicators
+# 25920 * sum_{l=1..3} int (p_k^(l))^2 over the cell for those same
+# cell-mean cubics (the reference's 720a^2 + 780b^2 + 781c^2 is exactly
+# that), stencil 2's second derivative again at node i+1, the downwind
+# stencil's indicator replaced by the largest of the five, eps = 25920 *
+# 1e-10 = 2.592e-6, and the weights d_k / (eps + beta_k) to the first power
+# (WENO7 and WENO5 square the denominator).  The five bandwidth-optimised
+# weights d_k are the source's
+# numbers (their sum is 1; the scheme's order is pinned by the convergence
+# test).  M is P on the mirrored stencil s[::-1].
+SYMBO_D = (0.0401954833730, 0.2493800006710, 0.4802686256260, 0.2009775476730, 0.0291783426580)
+
+
+def _symbo_face(s):
+    eps = Fr(2592, 10 ** 9)
+    q, beta = [], []
+    for k in range(5):
+        nodes = [k - 3 + m for m in range(4)]
+        vals = [Fr(float(x)) for x in s[k:k + 4]]
+        c = _cubic(nodes, vals, True)
+        q.append(sum(ci * Fr(1, 2) ** e for e, ci in enumerate(c)))
+        e2 = 1 if k == 2 else 0
+        beta.append(25920 * _smoothness([c[0], c[1], c[2] + 3 * c[3] * e2, c[3]]))
+    beta[4] = max(beta)
+    a = [Fr(dk) / (eps + b) for dk, b in zip(SYMBO_D, beta)]
+    return sum(x * y for x, y in zip(a, q)) / sum(a)
+
+
+def test_symbo_weights_sum_to_one():
+    assert abs(sum(SYMBO_D) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("side", [P, M])
+def test_weno7_symbo_face_value_from_first_principles(side):
+    for s in _weno7_inputs():
+        want = float(_symbo_face(s if side == P else s[::-1]))
+        got = O.face_flux(capi.SCHEME_IDS["WENO7_SYMBO"], side, s)
+        assert abs(got - want) <= 1e-13 * max(1.0, np.abs(s).max()) + 1e-15, (side, s, got, want)
