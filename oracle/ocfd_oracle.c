@@ -35,11 +35,12 @@
  * taken at node i+1, as src/OCFD_Schemes.cu:632 writes it), WENO5's the
  * same way (Jiang-Shu indicators, weights 1:6:3) and WENO7-SYMBO's
  * nonlinear branch (25920 x the integrals for the cell-mean cubics, the
- * downwind stencil at the largest indicator, first-power weights).  Parity
- * unpinned (restated from the source and checked against the HIP path
- * only): SYMBO's five optimised weights beyond their sum, the SYMBO TV limiter
- * (WENO_TV_Limiter, WENO_TV_MAX), the rand()-seeded IBC 108 and IBC 124 wall
- * blowing tables, the flag of hybrid variables 1..4 above, the filter's
+ * downwind stencil at the largest indicator, first-power weights) and the
+ * SYMBO TV limiter's switch between the linear and nonlinear weights.
+ * Parity unpinned (restated from the source and checked against the HIP
+ * path only): SYMBO's five optimised weights beyond their sum, the
+ * rand()-seeded IBC 108 and IBC 124 wall blowing tables, the flag of hybrid
+ * variables 1..4 above, the filter's
  * coverage geometry (which rows the reference's launch shapes reach), the
  * IBC 108 top-face start x_begin, and which node-to-node difference the
  * CHARTERIC boundary-node rules take.

@@ -39,6 +39,12 @@ value; the nonlinear weights must send the crossing stencils' weight to
 zero (ENO behaviour).  The OMP6 schemes' MP limiter must not create a new
 extremum (the value stays within the data range).  The linear UP7 on the
 same data misses by up to 0.43, which shows the test is sharp.
+
+WENO7, WENO5 and WENO7-SYMBO face values from first principles (at the end
+of this file): candidate values, smoothness indicators as integrals of the
+stencil polynomials' derivatives, optimal weights, formed in exact rational
+arithmetic, and the SYMBO TV limiter's switch between linear and nonlinear
+weights.
 """
 import math
 
@@ -457,7 +463,7 @@ def _symbo_face(s):
 
 
 def test_symbo_weights_sum_to_one():
-    assert abs(sum(SYMBO_D) - 1.0) < 1e-12
+    assert abs(sum(SYMBO_D) - 1.0) < 1e-11  # 13-digit decimals: 1 + 1e-12
 
 
 @pytest.mark.parametrize("side", [P, M])
@@ -466,3 +472,56 @@ def test_weno7_symbo_face_value_from_first_principles(side):
         want = float(_symbo_face(s if side == P else s[::-1]))
         got = O.face_flux(capi.SCHEME_IDS["WENO7_SYMBO"], side, s)
         assert abs(got - want) <= 1e-13 * max(1.0, np.abs(s).max()) + 1e-15, (side, s, got, want)
+
+
+def _symbo_linear(s):
+    """sum_k d_k q_k / sum_k d_k: the bandwidth-optimised linear scheme (the
+    weights normalised like the nonlinear ones; the source's d_k sum to
+    1 + 1e-12)."""
+    out = Fr(0)
+    for k in range(5):
+        c = _cubic([k - 3 + m for m in range(4)], [Fr(float(x)) for x in s[k:k + 4]], True)
+        out += Fr(SYMBO_D[k]) * sum(ci * Fr(1, 2) ** e for e, ci in enumerate(c))
+    return out / sum(Fr(d) for d in SYMBO_D)
+
+
+def _stencil_tv(s):
+    return [sum(abs(s[k + m + 1] - s[k + m]) for m in range(3)) for k in range(5)]
+
+
+def test_symbo_tv_limiter_switch():
+    """WENO7-SYMBO with the TV limiter (src/OCFD_Schemes.cu:391-478, lmt = 1):
+    when the largest total variation of the five stencils is below tv_max
+    and below tv_limiter times the smallest, the face value is the linear
+    scheme's; otherwise the nonlinear one.  Smooth data of small amplitude
+    take the linear branch, the same data scaled up (above tv_max) or with a
+    small kink (one flat stencil: ratio unbounded) the nonlinear one; the
+    M side never takes the linear branch (its smallest variation is never
+    formed, src/OCFD_Schemes.cu:513-524)."""
+    lim, tvm = 5.0, 0.2
+    j = np.arange(8)
+    cases = [0.01 * np.sin(0.3 * j + ph) for ph in (0.0, 0.9, 2.0)]
+    cases += [0.01 * np.exp(0.25 * j)]
+    sid = capi.SCHEME_IDS["WENO7_SYMBO_LIM"]
+    for s in cases:
+        tv = _stencil_tv(s)
+        assert max(tv) < tvm and max(tv) < lim * min(tv)
+        got = O.face_flux(sid, P, s, tv_limiter=lim, tv_max=tvm)
+        lin, nl = float(_symbo_linear(s)), float(_symbo_face(s))
+        assert abs(lin - nl) > 1e-9  # the two branches are told apart
+        assert abs(got - lin) <= 1e-15, (s, got, lin, nl)
+        big = 100.0 * s
+        assert max(_stencil_tv(big)) > tvm
+        assert abs(O.face_flux(sid, P, big, tv_limiter=lim, tv_max=tvm) - float(_symbo_face(big))) <= 1e-13
+        # either side of tv_max: the same shape with its largest variation
+        # at 0.95 and 1.25 tv_max
+        below, above = s * (0.95 * tvm / max(tv)), s * (1.25 * tvm / max(tv))
+        got = O.face_flux(sid, P, below, tv_limiter=lim, tv_max=tvm)
+        assert abs(got - float(_symbo_linear(below))) <= 1e-14, (s, got)
+        got = O.face_flux(sid, P, above, tv_limiter=lim, tv_max=tvm)
+        assert abs(got - float(_symbo_face(above))) <= 1e-14, (s, got)
+        kink = np.where(j > 4, s, s[4])
+        assert min(_stencil_tv(kink)) * lim <= max(_stencil_tv(kink))
+        assert abs(O.face_flux(sid, P, kink, tv_limiter=lim, tv_max=tvm) - float(_symbo_face(kink))) <= 1e-15
+        got_m = O.face_flux(sid, M, s, tv_limiter=lim, tv_max=tvm)
+        assert abs(got_m - float(_symbo_face(s[::-1]))) <= 1e-15, (s, got_m)
