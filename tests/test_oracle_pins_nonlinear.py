@@ -525,3 +525,20 @@ def test_symbo_tv_limiter_switch():
         assert abs(O.face_flux(sid, P, kink, tv_limiter=lim, tv_max=tvm) - float(_symbo_face(kink))) <= 1e-15
         got_m = O.face_flux(sid, M, s, tv_limiter=lim, tv_max=tvm)
         assert abs(got_m - float(_symbo_face(s[::-1]))) <= 1e-15, (s, got_m)
+
+
+def test_symbo_limiter_is_active_on_the_gpu_parity_case():
+    """The wavy box of the per-scheme GPU parity test (20x16x12, CD6) takes
+    the limiter's linear branch: its right-hand side with WENO7_SYMBO_LIM
+    differs from plain WENO7_SYMBO's by far more than the 1e-12 parity bar,
+    so that test checks the HIP path's switch as well."""
+    from pyocfd import cases
+    du = {}
+    for sch in ("WENO7_SYMBO", "WENO7_SYMBO_LIM"):
+        case = cases.wavy_box(20, 16, 12, scheme=sch, scheme_vis="CD6")
+        o = case.load(O.Oracle(case.cfg))
+        o.rhs(1)
+        du[sch] = np.asarray(o.get_du()).reshape(5, -1)
+    a, b = du["WENO7_SYMBO"], du["WENO7_SYMBO_LIM"]
+    rel = np.abs(a - b).max(1) / np.abs(a).max(1)
+    assert (rel > 1e-4).all(), rel
