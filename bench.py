@@ -166,7 +166,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported: this build runs on one GPU "
+        print(f"bench.py: --gpus {args.gpus} is not supported: the solver runs on one GPU "
               "(the multi-GPU halo exchange is a design only)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
