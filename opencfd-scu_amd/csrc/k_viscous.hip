@@ -45,13 +45,21 @@ namespace {
 #ifndef OCFD_VISC_TX
 #define OCFD_VISC_TX 32  // x columns per block
 #endif
-#ifndef OCFD_VISC_TY
-#define OCFD_VISC_TY 8  // y columns per block
+#ifndef OCFD_VISC_TY_D
+#define OCFD_VISC_TY_D 8  // y columns per block of the gradient kernel, FP64 path (16 measured 4.45 ->
+                          // 4.41 ms per launch on the 1024x384x256 swept ramp, the step 64.47 -> 64.18
+                          // ms, and passes the parity tests; 2 waves per SIMD either way)
+#endif
+#ifndef OCFD_VISC_TY_F
+#define OCFD_VISC_TY_F 8  // FP32 path: a 512-thread block would cost it its third wave per SIMD
 #endif
 #ifndef OCFD_VISC_PREFETCH
 #define OCFD_VISC_PREFETCH 1  // fetch plane k+1 while plane k is computed
 #endif
-constexpr int TX = OCFD_VISC_TX, TY = OCFD_VISC_TY;
+constexpr int TX = OCFD_VISC_TX;
+// y columns per block of the gradient/flux kernel
+template <typename S> struct TY_ { static constexpr int v = OCFD_VISC_TY_D; };
+template <> struct TY_<float> { static constexpr int v = OCFD_VISC_TY_F; };
 #ifndef OCFD_VDIV_KZ_D
 #define OCFD_VDIV_KZ_D 32  // z planes per block of the divergence, FP64 path: 32 against 16 on the
                            // 1024x384x256 swept ramp, the divergence's share of the step 13.65 ->
@@ -113,7 +121,7 @@ struct Tile25 {
     int i0, j0, i, j, kbeg, kend;
     bool own;
 };
-template <typename S, bool DIV, int TYB = TY>
+template <typename S, bool DIV, int TYB = TY_<S>::v>
 __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
     constexpr int KZ = KZ_<S, DIV>::v;
     Tile25 t;
@@ -135,7 +143,8 @@ __device__ __forceinline__ Tile25 tile_of(const Geo& g) {
 // false: no derivative of the block's plane reaches a closure (the caller
 // decides per block and plane), and the body is straight-line code.
 template <int SV, typename S, int PW, bool CL>
-__device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k, const S (*pl)[PW * (TY + 8)],
+__device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t, int k,
+                                         const S (*pl)[PW * (TY_<S>::v + 8)],
                                          const S (*q)[9], int tx, int ty, S rhx, S rhy, S rhz) {
     const Geo& g = A.g;
     // The point's metrics are loaded first, so that the loads are in flight
@@ -213,7 +222,7 @@ __device__ __forceinline__ void gf_point(const ViscArgsT<S>& A, const Tile25& t,
 
 // Whether a central-derivative closure is in reach of the block's plane k:
 // the same for every thread of the block.
-template <int TYB = TY, typename S>
+template <int TYB, typename S>
 __device__ __forceinline__ bool near_closure(const ViscArgsT<S>& A, const Tile25& t, int k) {
     const Geo& g = A.g;
     return (A.bnd[0] && t.i0 <= 3) || (A.bnd[1] && t.i0 + TX > g.nx - 4) || (A.bnd[2] && t.j0 <= 3) ||
@@ -224,10 +233,9 @@ __device__ __forceinline__ bool near_closure(const ViscArgsT<S>& A, const Tile25
 // LDS position and its global offset in plane 0 (-1: outside the ghost
 // layers, stored as 0).  Loads for plane k+1 are issued before plane k is
 // computed.
-constexpr int NT = TX * TY;
-
 template <int SV, typename S>
-__global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
+__global__ void __launch_bounds__(TX * TY_<S>::v) k_visc_gf(ViscArgsT<S> A) {
+    constexpr int TY = TY_<S>::v, NT = TX * TY;
     constexpr int PW = TX + 8, PN = PW * (TY + 8), NSL = (PN + NT - 1) / NT;
     const Geo& g = A.g;
     const Tile25 t = tile_of<S, false>(g);
@@ -344,7 +352,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
             load_strips(k + 5);
         }
         if (t.own) {
-            if (near_closure(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+            if (near_closure<TY>(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
             else gf_point<SV, S, PW, false>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
         }
         // plane k+5 into the slot plane k has just left
@@ -382,7 +390,7 @@ __global__ void __launch_bounds__(NT) k_visc_gf(ViscArgsT<S> A) {
             load_plane(k + 1);
         }
         if (t.own) {
-            if (near_closure(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
+            if (near_closure<TY>(A, t, k)) gf_point<SV, S, PW, true>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
             else gf_point<SV, S, PW, false>(A, t, k, pl, q, tx, ty, rhx, rhy, rhz);
         }
 #pragma unroll
@@ -566,7 +574,7 @@ __global__ void __launch_bounds__(256) k_qcrit(ViscArgsT<S> A, double* __restric
 
 template <typename S, bool DIV>
 dim3 grid25(const Geo& g) {
-    constexpr int KZ = KZ_<S, DIV>::v, TYB = DIV ? TYD : TY;
+    constexpr int KZ = KZ_<S, DIV>::v, TYB = DIV ? TYD : TY_<S>::v;
     return dim3((g.nx + TX - 1) / TX, (g.ny + TYB - 1) / TYB, (g.nz + KZ - 1) / KZ);
 }
 
@@ -574,8 +582,8 @@ dim3 grid25(const Geo& g) {
 
 template <typename S>
 void launch_visc_gf(const ViscArgsT<S>& a, hipStream_t st) {
-    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
-    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), (grid25<S, false>(a.g)), dim3(TX, TY), 0, st, a);
+    if (a.svis == OCFD_CD6) hipLaunchKernelGGL((k_visc_gf<OCFD_CD6, S>), (grid25<S, false>(a.g)), dim3(TX, TY_<S>::v), 0, st, a);
+    else hipLaunchKernelGGL((k_visc_gf<OCFD_CD8, S>), (grid25<S, false>(a.g)), dim3(TX, TY_<S>::v), 0, st, a);
 }
 // Flux components per divergence launch: two (measured on the 1024x384x256
 // swept ramp: FP64 3.00-3.30 ms per stage for two launches of two against
